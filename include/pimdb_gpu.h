@@ -1,0 +1,139 @@
+/*
+ * pimdb_gpu.h -- C ABI of the MI355X (gfx950) implementation of the PIMD-B++
+ * per-step hot path: bosonic exchange (Feldman-Hirshberg), pair and spring
+ * forces, and the OBABO step with a Langevin thermostat.
+ *
+ * All state of a ring polymer (P beads x N particles x D axes, FP64) stays
+ * resident in device memory.  Host arrays are caller-owned and copied; their
+ * layout is [bead][particle][axis], i.e. a stack of the reference's dVec
+ * (include/common.h:227-387).  Device arrays are bead-major SoA [bead][axis]
+ * [particle].  Units are atomic units, IPI convention (include/common.h:178-190).
+ *
+ * Errors: every call returns PIMDB_OK or an error code; pimdb_last_error()
+ * describes the last failure of a context.  PIMDB_E_NONFINITE mirrors the
+ * reference's std::overflow_error from the exchange recursions
+ * (src/bosonic_exchange/quadratic_bosonic_exchange.cpp:92-97, 119-124).
+ * A context is used by one host thread at a time and owns one HIP stream.
+ * No function falls back to a CPU path: a missing or unusable GPU is an
+ * error (PIMDB_E_HIP).
+ */
+#ifndef PIMDB_GPU_H
+#define PIMDB_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pimdb_status {
+    PIMDB_OK = 0,
+    PIMDB_E_NONFINITE = 1, /* exchange potential not finite (overflow_error) */
+    PIMDB_E_ARG = 2,       /* invalid argument (invalid_argument)            */
+    PIMDB_E_HIP = 3,       /* HIP runtime failure or no GPU                   */
+    PIMDB_E_STATE = 4      /* call not valid in the current state             */
+};
+
+enum pimdb_potential { PIMDB_POT_FREE = 0, PIMDB_POT_HARMONIC = 1, PIMDB_POT_DIPOLE = 2, PIMDB_POT_AZIZ = 3 };
+enum pimdb_propagator { PIMDB_PROP_CARTESIAN = 0, PIMDB_PROP_NORMAL_MODES = 1 };
+enum pimdb_thermostat { PIMDB_THERMO_NONE = 0, PIMDB_THERMO_LANGEVIN = 1 };
+
+/* One ring-polymer system.  Replaces the Params maps and the Simulation
+ * fields read from them (src/params.cpp:14-247, src/simulation.cpp:21-93).
+ * Quantities are already in atomic units (temperature in Hartree). */
+typedef struct pimdb_params {
+    int natoms, nbeads, ndim;  /* ndim 1..3                                  */
+    double mass;               /* electron masses                             */
+    double temperature;        /* Hartree (kB = 1)                            */
+    double dt, gamma;          /* time step, Langevin friction                */
+    int bosonic, pbc;          /* bosonic exchange on beads 1 and P; PBC       */
+    double size;               /* cubic box length (bohr)                      */
+    int ext_kind;              /* PIMDB_POT_FREE or PIMDB_POT_HARMONIC          */
+    double ext_omega;          /* trap frequency (energy units, hbar = 1)      */
+    int int_kind;              /* pimdb_potential                              */
+    double int_omega, int_strength, cutoff; /* cutoff < 0: all pairs           */
+    int propagator, thermostat, nmthermostat, fixcom;
+    unsigned int seed;         /* device noise stream seed                     */
+} pimdb_params;
+
+typedef struct pimdb_ctx pimdb_ctx;
+
+/* Creates a single-system context (the "plain" per-kernel form).
+ * Replaces Simulation::Simulation (src/simulation.cpp:16-101) minus
+ * initial-condition sampling, which stays host code. */
+int pimdb_ctx_create(const pimdb_params* params, pimdb_ctx** out);
+void pimdb_ctx_destroy(pimdb_ctx* ctx);
+/* Last failure of ctx; with ctx == NULL, the last failed create call of the
+ * calling thread (the reference's invalid_argument messages, params.cpp). */
+const char* pimdb_last_error(const pimdb_ctx* ctx);
+
+/* Host <-> device state, [bead][particle][axis].  Any pointer may be NULL.
+ * Replaces direct access to Simulation::coord/momenta/forces of every rank
+ * (include/simulation.h:59-60). */
+int pimdb_set_state(pimdb_ctx* ctx, const double* x, const double* p);
+int pimdb_get_state(pimdb_ctx* ctx, double* x, double* p, double* f);
+
+/* Forces on all beads from the current positions: exchange + interior
+ * springs + physical forces.  Replaces Simulation::updateForces on every rank
+ * (src/simulation.cpp:353-374) including BosonicExchange::prepare and
+ * exteriorSpringForce (bosonic_exchange_base.cpp:72-78). */
+int pimdb_forces(pimdb_ctx* ctx);
+
+/* Exchange forward pass from the current positions of beads 1 and P.
+ * VB = V[N] (effectivePotential, quadratic_bosonic_exchange.cpp:130-132);
+ * V, Vb: N+1 values (evaluateVBn / evaluateVBackwards, :73-128);
+ * prob: N x N row-major [l][u] (evaluateConnectionProbabilities, :142-157);
+ * E: N(N+1)/2 packed as the reference's E_kn (getEnk, :63-66).
+ * Any output may be NULL. */
+int pimdb_exchange_forward(pimdb_ctx* ctx, double* VB, double* V, double* Vb, double* prob, double* E);
+
+/* Exchange adjoint: exterior spring forces on beads 1 and P, N x D each
+ * (springForceFirstBead / springForceLastBead, :159-215), from the positions
+ * of beads 1, 2, P-1 and P.  Runs the forward pass itself. */
+int pimdb_exchange_adjoint(pimdb_ctx* ctx, double* f_first, double* f_last);
+
+/* nsteps OBABO steps (Simulation::run loop body, src/simulation.cpp:246-259).
+ * host_noise: NULL for on-device Philox4x32-10 noise, or the Gaussian deviates
+ * the reference's per-bead RanMars streams would draw, laid out
+ * [step][ostep(2)][bead][particle][axis] (langevin.cpp:18-26, in mode order
+ * under nmthermostat, thermostat_coupling.cpp:34-42).  Forces must be valid
+ * on entry (pimdb_forces, or zero as the reference has them before step 0). */
+int pimdb_step(pimdb_ctx* ctx, int nsteps, const double* host_noise);
+
+/* Observables summed over beads, atomic units (EnergyObservable,
+ * ClassicalObservable, BosonicObservable; src/observables/{energy,classical,
+ * bosonic}.cpp): out[10] = {kinetic, potential, ext_pot, int_pot, virial,
+ * temperature, cl_kinetic, cl_spring, prob_dist, prob_all}.  ext_pot/int_pot
+ * are 0 unless both potentials are non-free, as in energy.cpp:96-102. */
+int pimdb_observables(pimdb_ctx* ctx, double* out);
+
+/* Captures one MD step (device noise) in a hipGraph; later pimdb_step calls
+ * with host_noise == NULL replay it.  No reference counterpart (the
+ * reference has no graph capture). */
+int pimdb_graph_capture(pimdb_ctx* ctx);
+
+/* Average device time (ms) of `reps` launches of the physical-force kernel
+ * of one force evaluation, measured with HIP events on the context stream.
+ * No reference counterpart (the reference times only the run loop,
+ * src/simulation.cpp:225-226, 282-289). */
+int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_exchange, double* ms_step);
+
+/* ---------------------------------------------------------------------- */
+/* Exchange on caller-provided device arrays (the autograd harness).       */
+/* x1, xP: device N x D SoA [axis][particle].  Outputs are device arrays:  */
+/* VB (1), V (N+1), Vb (N+1), E (N(N+1)/2, may be NULL: then held inside). */
+/* grad: device [2][D][N] = -dV_B/dx1, -dV_B/dxP (the exchange forces      */
+/* without the interior links).  stream: a hipStream_t or NULL.            */
+/* No reference counterpart: the reference has no differentiable API.      */
+typedef struct pimdb_xplan pimdb_xplan;
+int pimdb_xplan_create(int natoms, int ndim, double beta_t, double spring_k, int pbc, double size,
+                       pimdb_xplan** out);
+void pimdb_xplan_destroy(pimdb_xplan* plan);
+int pimdb_xplan_forward(pimdb_xplan* plan, const double* x1, const double* xP, double* VB, double* V,
+                        double* Vb, void* stream);
+int pimdb_xplan_vb_grad(pimdb_xplan* plan, const double* x1, const double* xP, double* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+// device_common.hpp -- device helpers shared by the PIMD-B hot-path kernels
+// (gfx950, wave64).  All arithmetic is FP64.
+//
+// Bit-level notes: the distance helpers disable FP contraction so that a
+// min-imaged displacement, its squared norm and the norm are bit-identical to
+// the reference's (src/common.cpp:41-43, include/common.h:269-275); the
+// potentials follow the reference's operation order but let the compiler
+// contract, which moves results by at most a few ulp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pimdb {
+
+constexpr int WAVE = 64;
+
+enum PotKind : int { POT_FREE = 0, POT_HARMONIC = 1, POT_DIPOLE = 2, POT_AZIZ = 3 };
+
+// Aziz HFDHE2 constants, src/potentials/aziz.cpp:6-13 (atomic units).
+struct Aziz {
+    static constexpr double rm = 5.60738;
+    static constexpr double A = 0.5448504e6;
+    static constexpr double epsilon = 3.42016E-5;
+    static constexpr double alpha = 13.353384;
+    static constexpr double D = 1.241314;
+    static constexpr double C6 = 1.3732412;
+    static constexpr double C8 = 0.4253785;
+    static constexpr double C10 = 0.1781;
+};
+constexpr double HARD_CORE_EPS = 1.0E-7;   // include/common.h:193
+
+// dx - L floor(dx/L + 1/2), src/common.cpp:41-43.
+__device__ __forceinline__ double min_image(double dx, double L) {
+#pragma clang fp contract(off)
+    return dx - L * floor(dx / L + 0.5);
+}
+
+// Displacement b - a along one axis, min-imaged when pbc.
+__device__ __forceinline__ double disp(double a, double b, bool pbc, double L) {
+#pragma clang fp contract(off)
+    double d = b - a;
+    return pbc ? min_image(d, L) : d;
+}
+
+// Squared norm, summed over axes in order without contraction.
+template <int D>
+__device__ __forceinline__ double norm2(const double (&d)[D]) {
+#pragma clang fp contract(off)
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) s += d[a] * d[a];
+    return s;
+}
+
+// Scalar factor g(r) of the interaction gradient, gradV(d) = g * d
+// (harmonic.cpp:21-23, dipole.cpp:24-45, aziz.cpp:56-106).  `norm` is |d|.
+template <int KIND>
+__device__ __forceinline__ double pair_grad_factor(double norm, double kpair, double strength) {
+    if constexpr (KIND == POT_HARMONIC) {
+        return kpair;
+    } else if constexpr (KIND == POT_DIPOLE) {
+        return -3.0 * strength / (norm * norm * norm * norm * norm);
+    } else if constexpr (KIND == POT_AZIZ) {
+        const double r = norm / Aziz::rm;
+        const double T1 = -Aziz::A * Aziz::alpha * exp(-Aziz::alpha * r);
+        double g;
+        if (r > HARD_CORE_EPS && r < 0.01) {
+            g = T1 * (Aziz::epsilon / Aziz::rm);
+        } else {
+            const double ix = 1.0 / r;
+            const double ix2 = ix * ix;
+            const double ix6 = ix2 * ix2 * ix2;
+            const double ix7 = ix6 * ix;
+            const double ix8 = ix6 * ix2;
+            const double ix9 = ix8 * ix;
+            const double ix10 = ix8 * ix2;
+            const double ix11 = ix10 * ix;
+            double F = 1.0, dF = 0.0;
+            if (r < Aziz::D) {
+                // aziz.h:25-43: F uses D/x, dF uses D*(1/x); one exponential
+                // serves both (the two arguments differ by at most an ulp).
+                const double t = Aziz::D * ix - 1.0;
+                const double ex = exp(-t * t);
+                F = ex;
+                dF = 2.0 * Aziz::D * ix * ix * t * ex;
+            }
+            const double T2 = (6.0 * Aziz::C6 * ix7 + 8.0 * Aziz::C8 * ix9 + 10.0 * Aziz::C10 * ix11) * F;
+            const double T3 = -(Aziz::C6 * ix6 + Aziz::C8 * ix8 + Aziz::C10 * ix10) * dF;
+            g = (Aziz::epsilon / Aziz::rm) * (T1 + T2 + T3);
+        }
+        return g / norm;
+    } else {
+        return 0.0;
+    }
+}
+
+// Pair potential energy V(d) (harmonic.cpp:9-19, dipole.cpp:5-22, aziz.cpp:18-54).
+template <int KIND>
+__device__ __forceinline__ double pair_energy(double norm, double r2, double kpair, double strength) {
+    if constexpr (KIND == POT_HARMONIC) {
+        return 0.5 * kpair * r2;
+    } else if constexpr (KIND == POT_DIPOLE) {
+        return strength / (norm * norm * norm);
+    } else if constexpr (KIND == POT_AZIZ) {
+        const double r = norm / Aziz::rm;
+        const double Urep = Aziz::A * exp(-Aziz::alpha * r);
+        if (r > HARD_CORE_EPS && r < 0.01) return Aziz::epsilon * Urep;
+        const double ix2 = 1.0 / (r * r);
+        const double ix6 = ix2 * ix2 * ix2;
+        const double ix8 = ix6 * ix2;
+        const double ix10 = ix8 * ix2;
+        const double F = (r < Aziz::D) ? exp(-(Aziz::D / r - 1.0) * (Aziz::D / r - 1.0)) : 1.0;
+        const double Uatt = -(Aziz::C6 * ix6 + Aziz::C8 * ix8 + Aziz::C10 * ix10) * F;
+        return Aziz::epsilon * (Urep + Uatt);
+    } else {
+        return 0.0;
+    }
+}
+
+// ---- wave-level reductions (DPP/permute through the compiler's shuffles) ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, WAVE));
+    return v;
+}
+// Broadcast a double held by lane `src` to the whole wave.
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+    return __shfl(v, src, WAVE);
+}
+// Same for a wave-uniform lane index: two v_readlane into scalar registers.
+__device__ __forceinline__ double wave_bcast_uniform(double v, int src) {
+    const uint64_t bits = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)bits, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ----
+struct U4 { uint32_t x, y, z, w; };
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c.x;
+        const uint64_t p1 = (uint64_t)M1 * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Two standard normal deviates from one Philox block (Box-Muller on 53-bit
+// uniforms in (0,1]).  Counter = (element, step, ostep|stream, replica).
+__device__ __forceinline__ void gauss2(uint32_t elem, uint32_t step, uint32_t ostep,
+                                       uint32_t replica, uint32_t seed, double& g0, double& g1) {
+    const U4 r = philox4x32_10(U4{elem, step, ostep, replica}, seed, 0x5EEDB05Eu);
+    const double TWO53 = 9007199254740992.0;
+    const uint64_t a = ((uint64_t)r.x << 21) ^ (uint64_t)(r.y >> 11);
+    const uint64_t b = ((uint64_t)r.z << 21) ^ (uint64_t)(r.w >> 11);
+    const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) / TWO53;   // (0, 1]
+    const double u2 = (double)(b & ((1ull << 53) - 1)) / TWO53;           // [0, 1)
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    g0 = rad * c;
+    g1 = rad * s;
+}
+
+}  // namespace pimdb

@@ -1,0 +1,228 @@
+// integrator.hpp -- OBABO velocity Verlet, Langevin O-steps (Cartesian or
+// normal-mode coupling), the normal-mode propagator and centre-of-mass
+// momentum removal, on bead-major SoA arrays [b][a][i].
+//
+// Reference: Simulation::run (src/simulation.cpp:246-259), zeroMomentum
+// (:581-603), Propagator::momentStep/coordsStep and VelocityVerletPropagator
+// (src/propagators/velocity_verlet.cpp:7-38), NormalModesPropagator
+// (src/propagators/normal_modes_propagator.cpp:19-103), NormalModes
+// (src/normal_modes.cpp), LangevinThermostat (src/thermostats/langevin.cpp)
+// and the couplings (src/thermostats/thermostat_coupling.cpp).
+//
+// Momentum columns (all P beads of one (axis, particle)) are the unit of the
+// normal-mode transforms; a workgroup owns a tile of TC particles of one axis
+// for all beads, stages it in LDS and applies the P x P matrices there.  The
+// transforms are small (P <= 256) dense products kept on the FP64 VALU.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace pimdb {
+
+struct MomentaArgs {
+    int N, P, D, tc;            // tc: particles per tile
+    double half_dt;             // 0.5 * dt (momentStep, velocity_verlet.cpp:29)
+    double c1, c2;              // Langevin friction / noise (langevin.cpp:10-13)
+    int thermostat;             // 0 none, 1 langevin
+    int nm;                     // normal-mode coupling (thermostat_coupling.cpp:19-47)
+    const double* C;            // [P][P] cart->nm, rows as built per rank
+    const double* CT;           // [P][P] nm->cart
+    const double* noise;        // host noise [b][a][i] for this O-step, or null (Philox)
+    const unsigned long long* step_ctr;
+    uint32_t ostep, seed, replica;
+    const double* com_in;       // partial momentum sums to subtract first, or null
+    int ncom_in;                // number of partial rows [ncom_in][D]
+    double* com_out;            // partial sums of the result, [gridDim.x][D], or null
+    const double* kick_f;       // forces for a half kick first, or null
+};
+
+__device__ __forceinline__ double com_value(const double* part, int npart, int D, int a, int N, int P) {
+    double s = 0.0;
+    for (int t = 0; t < npart; ++t) s += part[(size_t)t * D + a];
+    return s / (double)(N * P);
+}
+
+__device__ __forceinline__ double noise_at(const MomentaArgs& m, int b, int a, int i) {
+    const size_t e = ((size_t)b * m.D + a) * m.N + i;
+    if (m.noise) return m.noise[e];
+    double g0, g1;
+    gauss2((uint32_t)e, (uint32_t)*m.step_ctr, m.ostep, m.replica, m.seed, g0, g1);
+    return g0;
+}
+
+// Optional COM subtraction, optional half kick, optional O-step, optional
+// partial COM sums of the result.  Grid (ceil(N/tc), D).
+static __global__ void __launch_bounds__(256) k_momenta(MomentaArgs m, double* __restrict__ p) {
+    extern __shared__ double smem[];
+    const int P = m.P, N = m.N, D = m.D, TC = m.tc;
+    double* pc = smem;                       // [P][TC]
+    double* pn = smem + (size_t)P * TC;      // [P][TC]
+    double* red = pn + (size_t)P * TC;       // [blockDim/WAVE]
+    const int a = blockIdx.y;
+    const int i0 = blockIdx.x * TC;
+    const int tid = threadIdx.x;
+    const int ncol = min(TC, N - i0);
+    const double cm = m.com_in ? com_value(m.com_in, m.ncom_in, D, a, N, P) : 0.0;
+    for (int e = tid; e < P * TC; e += blockDim.x) {
+        const int b = e / TC, c = e % TC;
+        double v = 0.0;
+        if (c < ncol) {
+            const size_t g = ((size_t)b * D + a) * N + i0 + c;
+            v = p[g];
+            if (m.com_in) v -= cm;
+            if (m.kick_f) v += m.half_dt * m.kick_f[g];
+        }
+        pc[e] = v;
+    }
+    if (m.thermostat == 1) {
+        if (!m.nm) {
+            for (int e = tid; e < P * TC; e += blockDim.x) {
+                const int b = e / TC, c = e % TC;
+                if (c < ncol) pc[e] = m.c1 * pc[e] + m.c2 * noise_at(m, b, a, i0 + c);
+            }
+        } else {
+            __syncthreads();
+            // mode k of column c: sum_j C[k][j] p_j, then the Langevin update
+            for (int e = tid; e < P * TC; e += blockDim.x) {
+                const int k = e / TC, c = e % TC;
+                double s = 0.0;
+                const double* row = m.C + (size_t)k * P;
+                for (int j = 0; j < P; ++j) s += row[j] * pc[j * TC + c];
+                pn[e] = (c < ncol) ? m.c1 * s + m.c2 * noise_at(m, k, a, i0 + c) : 0.0;
+            }
+            __syncthreads();
+            for (int e = tid; e < P * TC; e += blockDim.x) {
+                const int b = e / TC, c = e % TC;
+                double s = 0.0;
+                const double* row = m.CT + (size_t)b * P;
+                for (int k = 0; k < P; ++k) s += row[k] * pn[k * TC + c];
+                pc[e] = s;
+            }
+        }
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int e = tid; e < P * TC; e += blockDim.x) {
+        const int b = e / TC, c = e % TC;
+        if (c < ncol) {
+            p[((size_t)b * D + a) * N + i0 + c] = pc[e];
+            part += pc[e];
+        }
+    }
+    if (m.com_out) {
+        part = wave_sum(part);
+        if (tid % WAVE == 0) red[tid / WAVE] = part;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < (int)blockDim.x / WAVE; ++w) s += red[w];
+            m.com_out[(size_t)blockIdx.x * D + a] = s;
+        }
+    }
+}
+
+struct DriftArgs {
+    int N, P, D;
+    double half_dt, dt, mass;
+    const double* com_in;
+    int ncom_in;
+    unsigned long long* step_ctr;   // incremented once per call when non-null
+};
+
+// p -= p_cm; p += dt/2 f; x += dt p / m  (velocity_verlet.cpp:24-38)
+static __global__ void __launch_bounds__(256) k_kick_drift(DriftArgs d, double* __restrict__ x, double* __restrict__ p,
+                                                    const double* __restrict__ f) {
+    const size_t tot = (size_t)d.P * d.D * d.N;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0 && d.step_ctr) *d.step_ctr += 1;
+    if (e >= tot) return;
+    const int a = (int)((e / d.N) % d.D);
+    double pv = p[e];
+    if (d.com_in) pv -= com_value(d.com_in, d.ncom_in, d.D, a, d.N, d.P);
+    pv += d.half_dt * f[e];
+    p[e] = pv;
+    x[e] += d.dt * pv / d.mass;
+}
+
+// p -= p_cm (the deferred zeroMomentum of the last O-step)
+static __global__ void __launch_bounds__(256) k_com_subtract(int N, int P, int D, const double* __restrict__ com_in,
+                                                      int ncom_in, double* __restrict__ p) {
+    const size_t tot = (size_t)P * D * N;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= tot) return;
+    const int a = (int)((e / N) % D);
+    p[e] -= com_value(com_in, ncom_in, D, a, N, P);
+}
+
+struct NmPropArgs {
+    int N, P, D, tc;
+    double half_dt, dt, mass;
+    const double* C;
+    const double* CT;
+    const double* rot;      // [P][4]: cos(w dt), sin(w dt), m w, w
+    const double* com_in;
+    int ncom_in;
+    const double* fext;     // external forces of the previous evaluation
+    unsigned long long* step_ctr;
+};
+
+// Normal-mode propagator for distinguishable particles: external half kick,
+// exact free ring-polymer evolution in normal modes, back to Cartesian
+// (normal_modes_propagator.cpp:19-53).  Grid (ceil(N/tc), D).
+static __global__ void __launch_bounds__(256) k_nm_propagate(NmPropArgs m, double* __restrict__ x, double* __restrict__ p) {
+    extern __shared__ double smem[];
+    const int P = m.P, N = m.N, D = m.D, TC = m.tc;
+    double* xc = smem;
+    double* pc = xc + (size_t)P * TC;
+    double* qn = pc + (size_t)P * TC;
+    double* pn = qn + (size_t)P * TC;
+    const int a = blockIdx.y;
+    const int i0 = blockIdx.x * TC;
+    const int tid = threadIdx.x;
+    const int ncol = min(TC, N - i0);
+    if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && m.step_ctr) *m.step_ctr += 1;
+    const double cm = m.com_in ? com_value(m.com_in, m.ncom_in, D, a, N, P) : 0.0;
+    for (int e = tid; e < P * TC; e += blockDim.x) {
+        const int b = e / TC, c = e % TC;
+        double xv = 0.0, pv = 0.0;
+        if (c < ncol) {
+            const size_t g = ((size_t)b * D + a) * N + i0 + c;
+            xv = x[g];
+            pv = p[g];
+            if (m.com_in) pv -= cm;
+            pv += m.half_dt * m.fext[g];
+        }
+        xc[e] = xv;
+        pc[e] = pv;
+    }
+    __syncthreads();
+    for (int e = tid; e < P * TC; e += blockDim.x) {
+        const int k = e / TC, c = e % TC;
+        const double* row = m.C + (size_t)k * P;
+        double q = 0.0, pp = 0.0;
+        for (int j = 0; j < P; ++j) q += row[j] * xc[j * TC + c];
+        for (int j = 0; j < P; ++j) pp += row[j] * pc[j * TC + c];
+        const double cs = m.rot[4 * k], sn = m.rot[4 * k + 1], mw = m.rot[4 * k + 2], w = m.rot[4 * k + 3];
+        if (w == 0) {
+            qn[e] = q + m.dt / m.mass * pp;
+            pn[e] = pp;
+        } else {
+            qn[e] = cs * q + sn / mw * pp;
+            pn[e] = (-1) * mw * sn * q + cs * pp;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < P * TC; e += blockDim.x) {
+        const int b = e / TC, c = e % TC;
+        if (c >= ncol) continue;
+        const double* row = m.CT + (size_t)b * P;
+        double xv = 0.0, pv = 0.0;
+        for (int k = 0; k < P; ++k) xv += row[k] * qn[k * TC + c];
+        for (int k = 0; k < P; ++k) pv += row[k] * pn[k * TC + c];
+        const size_t g = ((size_t)b * D + a) * N + i0 + c;
+        x[g] = xv;
+        p[g] = pv;
+    }
+}
+
+}  // namespace pimdb

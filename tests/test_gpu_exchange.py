@@ -1,0 +1,134 @@
+"""HIP exchange (forward V, Vb, probabilities, E; adjoint forces) against the
+oracle, through the C ABI.  Tolerances (FP64): V, Vb norm-wise relative
+1e-10; probabilities absolute 1e-10; exterior forces norm-wise relative
+1e-10 per bead."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pimdb_amd as G
+from helpers import KELVIN, aziz_config, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def base_kw(N, P, D, pbc, L=8.0, T=0.02, mass=1.0):
+    return dict(natoms=N, nbeads=P, ndim=D, mass=mass, temperature=T, dt=1.0, gamma=0.01,
+                bosonic=1, pbc=int(pbc), size=L, ext_kind="harmonic", ext_omega=0.05,
+                int_kind="free", propagator="cartesian", thermostat="langevin", seed=11)
+
+
+def run_both(kw, x):
+    ctx = G.Context(G.make_params(**kw))
+    ctx.set_state(x)
+    g = ctx.exchange_forward()
+    f1, fP = ctx.exchange_adjoint()
+    ctx.close()
+    d = O.derived(O.make_params(**kw))
+    P = kw["nbeads"]
+    o = O.exchange(x[0], x[1 % P], x[(P - 2) % P], x[P - 1], d["beta_t"], d["k"], kw["pbc"], kw["size"])
+    return g, (f1, fP), o
+
+
+def check(g, f, o, N):
+    assert rel_err(g["E"], o["E"]) < 1e-12
+    assert rel_err(g["V"], o["V"]) < 1e-10
+    assert rel_err(g["Vb"], o["Vb"]) < 1e-10
+    assert abs(g["VB"] - o["V"][N]) <= 1e-10 * max(1.0, abs(o["V"][N]))
+    assert np.max(np.abs(g["prob"] - o["prob"])) < 1e-10
+    assert rel_err(f[0], o["f_first"]) < 1e-10
+    assert rel_err(f[1], o["f_last"]) < 1e-10
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 7, 31, 32, 33, 64, 65, 100])
+@pytest.mark.parametrize("D,pbc", [(3, False), (3, True), (2, False), (1, True)])
+def test_exchange_random_rings(N, D, pbc):
+    P = 6
+    rng = np.random.default_rng(1000 * N + 10 * D + pbc)
+    x = rng.uniform(-3.0, 3.0, size=(P, N, D))
+    kw = base_kw(N, P, D, pbc, L=6.0)
+    g, f, o = run_both(kw, x)
+    check(g, f, o, N)
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_exchange_few_beads(P):
+    N, D = 9, 3
+    rng = np.random.default_rng(P)
+    x = rng.uniform(-2.0, 2.0, size=(P, N, D))
+    g, f, o = run_both(base_kw(N, P, D, False), x)
+    check(g, f, o, N)
+
+
+@pytest.mark.parametrize("replicate", [True, False])
+def test_exchange_headline_aziz(replicate):
+    """N=256 He-4, P=64, T=1.5 K, PBC at 0.02186 A^-3."""
+    kw, x = aziz_config(256, 64, 1.5, replicate=replicate, seed=12345)
+    g, f, o = run_both(kw, x)
+    check(g, f, o, 256)
+    P = g["prob"]
+    assert np.allclose(P.sum(axis=1), 1.0, atol=1e-10)
+    assert np.allclose(P.sum(axis=0), 1.0, atol=1e-10)
+
+
+def test_exchange_large_N1024_thermal():
+    """N=1024, P=128, T=1.2 K (the large configuration); beads spread around
+    the particle positions on a thermal scale."""
+    kw, x = aziz_config(1024, 128, 1.2, replicate=True, seed=777)
+    x = x + np.random.default_rng(778).normal(0, 1.0, size=x.shape)
+    g, f, o = run_both(kw, x)
+    check(g, f, o, 1024)
+
+
+def test_exchange_large_N1024_ill_conditioned():
+    """Independent uniform beads at N=1024: V[N] ~ 660 Ha, beta~ V ~ 1.4e6, so
+    one ulp of any V moves a probability by ~beta~ eps |V| ~ 1.5e-10
+    relative; the reference's own sums carry that error.  Tolerance:
+    64 beta~ eps max|V| (probabilities absolute, forces norm-wise)."""
+    kw, x = aziz_config(1024, 128, 1.2, replicate=False, seed=777)
+    g, f, o = run_both(kw, x)
+    bt = O.derived(O.make_params(**kw))["beta_t"]
+    tol = 64 * bt * np.finfo(float).eps * np.max(np.abs(o["V"]))
+    assert tol < 1e-7
+    assert rel_err(g["V"], o["V"]) < 1e-12
+    assert np.max(np.abs(g["prob"] - o["prob"])) < tol
+    assert rel_err(f[0], o["f_first"]) < tol
+    assert rel_err(f[1], o["f_last"]) < tol
+
+
+def test_exchange_thermal_spread():
+    """Beads 1 and P displaced by thermal-scale offsets (many exchange paths
+    contribute): the most demanding case for the shifted sums."""
+    N, P, D = 128, 32, 3
+    kw = base_kw(N, P, D, True, L=20.0, T=2.0 * KELVIN, mass=4.0026 * 1822.8885)
+    rng = np.random.default_rng(3)
+    x0 = rng.uniform(-10, 10, size=(N, D))
+    x = np.repeat(x0[None], P, axis=0) + rng.normal(0, 1.0, size=(P, N, D))
+    g, f, o = run_both(kw, x)
+    check(g, f, o, N)
+
+
+def test_nonfinite_positions_raise_overflow():
+    N, P, D = 8, 4, 3
+    x = np.zeros((P, N, D))
+    x[0, 3, 1] = np.nan
+    ctx = G.Context(G.make_params(**base_kw(N, P, D, False)))
+    ctx.set_state(x)
+    with pytest.raises(G.NonFiniteError):
+        ctx.exchange_forward()
+    with pytest.raises(OverflowError):
+        ctx.forces()
+    # the context stays usable afterwards
+    x[0, 3, 1] = 0.5
+    ctx.set_state(x)
+    ctx.exchange_forward()
+    ctx.close()
+
+
+def test_exchange_requires_bosonic():
+    kw = base_kw(5, 4, 3, False)
+    kw["bosonic"] = 0
+    ctx = G.Context(G.make_params(**kw))
+    with pytest.raises(G.PimdbError) as ei:
+        ctx.exchange_forward()
+    assert ei.value.code == G.E_STATE

@@ -1,0 +1,65 @@
+"""Energy / classical / bosonic observables computed on the GPU against the
+oracle (summed over beads, atomic units) and against the reference's
+simulation.out rows."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pimdb_amd as G
+from helpers import KELVIN, aziz_config, case_params, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["kinetic", "potential", "ext_pot", "int_pot", "virial", "temperature", "cl_kinetic",
+         "cl_spring", "prob_dist", "prob_all"]
+
+
+def compare(kw, x, p, rtol=1e-9):
+    sim = O.Sim(O.make_params(**kw))
+    sim.set_state(x, p)
+    ctx = G.Context(G.make_params(**kw))
+    ctx.set_state(x, p)
+    og = ctx.observables()
+    oo = sim.observables()
+    ctx.close()
+    scale = np.maximum(np.abs(oo), 1e-300)
+    for j, n in enumerate(NAMES):
+        if n in ("prob_dist", "prob_all"):
+            assert abs(og[j] - oo[j]) <= 1e-10 + rtol * abs(oo[j]), (n, og[j], oo[j])
+        else:
+            assert abs(og[j] - oo[j]) <= rtol * max(scale[j], 1e-12), (n, og[j], oo[j])
+
+
+@pytest.mark.parametrize("replicate", [True, False])
+def test_observables_aziz(replicate):
+    kw, x = aziz_config(96, 16, 1.5, replicate=replicate, seed=31)
+    p = np.random.default_rng(1).normal(0, 0.3, size=x.shape)
+    compare(kw, x, p)
+
+
+@pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic", "dist_harmonic",
+                                  "bosonic_quadratic_harmonic_nmthermostat_dynamics"])
+def test_observables_reference_rows(case):
+    """Steps on the GPU (reference noise) then its observables against the
+    stored simulation.out rows (printed with 9 significant digits)."""
+    kw, _ = case_params(case)
+    fx = load_fixture(case)
+    header = [str(h) for h in fx["sim_out_header"]]
+    sim = O.Sim(O.make_params(**kw))
+    sim.init_random()
+    x, p, _ = sim.state()
+    ctx = G.Context(G.make_params(**kw))
+    ctx.set_state(x, p)
+    done = 0
+    for row in fx["sim_out"][:4]:
+        n = int(row[0]) + 1 - done
+        noise = np.stack([sim.draw_noise() for _ in range(n)])
+        ctx.step(n, noise)
+        done += n
+        ob = ctx.observables() / KELVIN
+        for j, name in enumerate(header[1:]):
+            if name in ("ext_pot", "int_pot"):
+                continue
+            ref = row[1 + j]
+            assert abs(ob[NAMES.index(name)] - ref) <= 2e-8 * abs(ref) + 1e-8, (name, ob, ref)
+    ctx.close()

@@ -1,0 +1,251 @@
+"""The oracle against what the mathematics fixes: brute-force enumeration of
+V_B, closed forms, invariants of the connection probabilities, finite
+differences of V_B and of the pair potentials, image enumeration for the
+minimum image and the cutoff, and the normal-mode transform."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import ANGSTROM, KELVIN
+
+
+def ring(N, D, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(0, scale, size=(N, D)) for _ in range(4)]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 8, 12])
+@pytest.mark.parametrize("pbc", [False, True])
+def test_vb_matches_composition_enumeration(N, pbc):
+    x1, x2, xPm1, xP = ring(N, 3, 100 + N, scale=1.5)
+    bt, k = 2.0, 0.7
+    out = O.exchange(x1, x2, xPm1, xP, bt, k, pbc, 4.0)
+    vb = O.vb_enumerate(out["E"], N, bt)
+    assert abs(out["V"][N] - vb) <= 1e-12 * max(1.0, abs(vb))
+
+
+def test_cycle_energy_closed_form():
+    N, D = 9, 2
+    x1, _, _, xP = ring(N, D, 5)
+    k = 1.3
+    E = O.cycle_energies(x1, xP, k)
+    for v in range(N):
+        for u in range(v + 1):
+            s = sum(np.sum((x1[j + 1] - xP[j]) ** 2) for j in range(u, v)) + np.sum((x1[u] - xP[v]) ** 2)
+            m, kk = v + 1, v - u + 1
+            assert abs(E[m * (m + 1) // 2 - kk] - 0.5 * k * s) < 1e-12 * max(1, 0.5 * k * s)
+
+
+def test_closed_forms_N1_N2():
+    k, bt = 0.9, 1.7
+    x1, x2, xPm1, xP = ring(1, 3, 1)
+    out = O.exchange(x1, x2, xPm1, xP, bt, k)
+    assert abs(out["V"][1] - 0.5 * k * np.sum((x1 - xP) ** 2)) < 1e-14
+    x1, x2, xPm1, xP = ring(2, 3, 2)
+    out = O.exchange(x1, x2, xPm1, xP, bt, k)
+    E11 = 0.5 * k * np.sum((x1[0] - xP[0]) ** 2)
+    E22 = 0.5 * k * np.sum((x1[1] - xP[1]) ** 2)
+    E12 = 0.5 * k * (np.sum((x1[1] - xP[0]) ** 2) + np.sum((x1[0] - xP[1]) ** 2))
+    vb = -1.0 / bt * math.log(0.5 * (math.exp(-bt * (E11 + E22)) + math.exp(-bt * E12)))
+    assert abs(out["V"][2] - vb) < 1e-13
+
+
+@pytest.mark.parametrize("N", [2, 5, 17])
+def test_coincident_beads(N):
+    """All beads of all particles at one point: every cycle energy is zero."""
+    rng = np.random.default_rng(N)
+    x = np.repeat(rng.normal(0, 1, size=(1, 3)), N, axis=0)
+    out = O.exchange(x, x, x, x, 3.0, 1.1)
+    assert np.max(np.abs(out["V"])) < 1e-13
+    P = out["prob"]
+    for l in range(N - 1):
+        assert np.allclose(P[l, :l + 1], 1.0 / ((l + 1) * (l + 2)), rtol=1e-13, atol=0)
+        assert abs(P[l, l + 1] - (l + 1) / (l + 2)) < 1e-13
+    assert np.allclose(P[N - 1], 1.0 / N, rtol=1e-13, atol=0)
+
+
+def test_distinguishable_limit():
+    N, bt, k = 6, 1.0, 1.0
+    rng = np.random.default_rng(3)
+    centres = np.arange(N)[:, None] * np.array([[40.0, 0.0, 0.0]])
+    x1 = centres + rng.normal(0, 0.1, size=(N, 3))
+    xP = centres + rng.normal(0, 0.1, size=(N, 3))
+    out = O.exchange(x1, x1, xP, xP, bt, k)
+    expect = sum(0.5 * k * np.sum((x1[i] - xP[i]) ** 2) for i in range(N)) + math.lgamma(N + 1) / bt
+    assert abs(out["V"][N] - expect) < 1e-10
+    assert np.allclose(out["prob"], np.eye(N), atol=1e-12)
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_probability_sums(pbc):
+    N = 11
+    x1, x2, xPm1, xP = ring(N, 3, 9, scale=0.8)
+    out = O.exchange(x1, x2, xPm1, xP, 1.3, 0.8, pbc, 3.0)
+    P = out["prob"]
+    assert np.allclose(P.sum(axis=1), 1.0, atol=1e-12)
+    assert np.allclose(P.sum(axis=0), 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_exchange_forces_are_minus_gradient(pbc):
+    """F1, FP = -grad(V_B + k/2|x2-x1|^2 + k/2|x_{P-1}-x_P|^2) by central FD
+    (independent of the probability formulas)."""
+    N, D, bt, k, L = 6, 3, 1.1, 0.9, 5.0
+    x1, x2, xPm1, xP = ring(N, D, 21, scale=1.2)
+
+    def energy(a1, aP):
+        V = O.exchange(a1, x2, xPm1, aP, bt, k, pbc, L)["V"][N]
+        d1 = x2 - a1
+        dP = xPm1 - aP
+        if pbc:
+            d1 = d1 - L * np.floor(d1 / L + 0.5)
+            dP = dP - L * np.floor(dP / L + 0.5)
+        return V + 0.5 * k * np.sum(d1 ** 2) + 0.5 * k * np.sum(dP ** 2)
+
+    out = O.exchange(x1, x2, xPm1, xP, bt, k, pbc, L)
+    h = 1e-6
+    for which, f in ((0, out["f_first"]), (1, out["f_last"])):
+        for i in range(N):
+            for a in range(D):
+                xa, xb = x1.copy(), xP.copy()
+                tgt = xa if which == 0 else xb
+                tgt[i, a] += h
+                ep = energy(xa, xb)
+                tgt[i, a] -= 2 * h
+                em = energy(xa, xb)
+                fd = -(ep - em) / (2 * h)
+                assert abs(fd - f[i, a]) < 2e-7 * max(1.0, abs(f[i, a])), (which, i, a, fd, f[i, a])
+
+
+def _grad(kind, d, **kw):
+    L = O.lib()
+    g = np.zeros(len(d))
+    d = np.ascontiguousarray(d, dtype=float)
+    if kind == "aziz":
+        L.orc_aziz_grad(len(d), O.ptr(d), O.ptr(g))
+    elif kind == "dipole":
+        L.orc_dipole_grad(len(d), kw["s"], O.ptr(d), O.ptr(g))
+    else:
+        L.orc_harmonic_grad(len(d), kw["m"], kw["w"], O.ptr(d), O.ptr(g))
+    return g
+
+
+def _pot(kind, d, **kw):
+    L = O.lib()
+    d = np.ascontiguousarray(d, dtype=float)
+    if kind == "aziz":
+        return L.orc_aziz_V(len(d), O.ptr(d))
+    if kind == "dipole":
+        return L.orc_dipole_V(len(d), kw["s"], O.ptr(d))
+    return L.orc_harmonic_V(len(d), kw["m"], kw["w"], O.ptr(d))
+
+
+@pytest.mark.parametrize("r", [0.04, 2.5, 4.0, 5.60738, 6.9, 9.0, 15.0])
+def test_aziz_gradient_matches_fd(r):
+    rng = np.random.default_rng(int(r * 100))
+    u = rng.normal(size=3)
+    d = r * u / np.linalg.norm(u)
+    g = _grad("aziz", d)
+    h = 1e-6 * max(r, 1.0)
+    for a in range(3):
+        dp, dm = d.copy(), d.copy()
+        dp[a] += h
+        dm[a] -= h
+        fd = (_pot("aziz", dp) - _pot("aziz", dm)) / (2 * h)
+        assert abs(fd - g[a]) <= 1e-6 * np.max(np.abs(g)) + 1e-14
+
+
+def test_aziz_well_depth():
+    """HFDHE2: V(r_m) ~ -epsilon (10.8 K at 2.9673 angstrom)."""
+    eps = 3.42016e-5
+    v = _pot("aziz", np.array([5.60738, 0.0, 0.0]))
+    assert abs(v / eps + 1.0) < 1e-3
+    assert abs(v / KELVIN + 10.8) < 0.02
+    assert abs(5.60738 / ANGSTROM - 2.9673) < 1e-4
+
+
+def test_harmonic_and_dipole_closed_forms():
+    d = np.array([0.3, -1.2, 2.0])
+    r = np.linalg.norm(d)
+    assert np.allclose(_grad("harmonic", d, m=3.0, w=0.5), 3.0 * 0.25 * d, rtol=1e-15)
+    assert abs(_pot("harmonic", d, m=3.0, w=0.5) - 0.5 * 0.75 * r * r) < 1e-14
+    assert np.allclose(_grad("dipole", d, s=2.5), -3 * 2.5 / r ** 5 * d, rtol=1e-14)
+    assert abs(_pot("dipole", d, s=2.5) - 2.5 / r ** 3) < 1e-14
+
+
+def test_min_image_brute_force():
+    rng = np.random.default_rng(0)
+    L = 7.3
+    for dx in rng.uniform(-40, 40, size=2000):
+        m = O.lib().orc_min_image(dx, L)
+        best = min((abs(dx - n * L) for n in range(-10, 11)))
+        assert abs(abs(m) - best) < 1e-12
+        assert -0.5 * L <= m <= 0.5 * L
+
+
+@pytest.mark.parametrize("kind", ["aziz", "dipole", "harmonic"])
+def test_pair_forces_image_enumeration(kind):
+    """Physical forces with PBC and cutoff L/2 against an explicit search of
+    the nearest image of every pair and the cutoff rule |r| < c."""
+    N, D, L = 7, 3, 14.0
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-L, L, size=(N, D))
+    kw = dict(natoms=N, nbeads=1, ndim=D, mass=2.0, temperature=1.0, dt=1.0, gamma=0.0,
+              bosonic=0, pbc=1, size=L, ext_kind="harmonic", ext_omega=0.03,
+              int_kind=kind, int_omega=0.02, int_strength=0.7, cutoff=100.0)  # clamped to L/2
+    p = O.make_params(**kw)
+    f = O.physical_forces(p, x)
+    ref = -2.0 * 0.03 ** 2 * x
+    shifts = [np.array(s) for s in np.ndindex(5, 5, 5)]
+    for i in range(N):
+        for j in range(N):
+            if i == j:
+                continue
+            cands = [x[i] - x[j] - L * (s - 2) for s in shifts]
+            d = min(cands, key=lambda c: np.linalg.norm(c))
+            if np.linalg.norm(d) < 0.5 * L:
+                ref[i] -= _grad(kind, d, s=0.7, m=2.0, w=0.02)
+    assert np.max(np.abs(f - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 8, 9, 64])
+def test_nm_matrices_orthogonal(P):
+    Cm, CT = O.nm_matrices(P)
+    assert np.allclose(Cm @ Cm.T, np.eye(P), atol=1e-12)
+    assert np.allclose(CT, Cm.T, atol=1e-12)
+
+
+def test_nm_propagator_free_ring_is_exact():
+    """Free distinguishable ring, no thermostat: every normal mode rotates
+    exactly at w_k = 2 w_P sin(k pi/P) (normal_modes_propagator.cpp:13-39)."""
+    N, P, D = 3, 8, 3
+    kw = dict(natoms=N, nbeads=P, ndim=D, mass=5.0, temperature=0.01, dt=2.0, gamma=0.0,
+              bosonic=0, pbc=0, size=10.0, ext_kind="free", int_kind="free",
+              propagator="normal_modes", thermostat="none", fixcom=0, seed=5)
+    p = O.make_params(**kw)
+    rng = np.random.default_rng(1)
+    x0 = rng.normal(0, 1, size=(P, N, D))
+    p0 = rng.normal(0, 1, size=(P, N, D))
+    sim = O.Sim(p)
+    sim.set_state(x0, p0)
+    nsteps = 50
+    for _ in range(nsteps):
+        sim.step()
+    x, pm, _ = sim.state()
+    Cm, _ = O.nm_matrices(P)
+    q0 = np.einsum("kj,jnd->knd", Cm, x0)
+    v0 = np.einsum("kj,jnd->knd", Cm, p0)
+    beta = 1.0 / 0.01
+    wp = P / beta
+    t = nsteps * 2.0
+    qt = np.empty_like(q0)
+    for k in range(P):
+        w = 2 * wp * math.sin(k * math.pi / P)
+        if w == 0:
+            qt[k] = q0[k] + t / 5.0 * v0[k]
+        else:
+            qt[k] = math.cos(w * t) * q0[k] + math.sin(w * t) / (5.0 * w) * v0[k]
+    xt = np.einsum("kj,knd->jnd", Cm, qt)
+    assert np.max(np.abs(x - xt)) < 1e-10
