@@ -118,6 +118,12 @@ int pimdb_graph_capture(pimdb_ctx* ctx);
  * src/simulation.cpp:225-226, 282-289). */
 int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_exchange, double* ms_step);
 
+/* Micro-benchmarks for the roofline: FP64 FMA rate (TFLOP/s), FP64 exp
+ * throughput of the device math library (Gexp/s) and HBM copy bandwidth
+ * (GB/s, read + write, over copy_bytes; <= 0 means 1 GiB).  Any output may
+ * be NULL.  No reference counterpart. */
+int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gbs, long long copy_bytes);
+
 /* ---------------------------------------------------------------------- */
 /* Exchange on caller-provided device arrays (the autograd harness).       */
 /* x1, xP: device N x D SoA [axis][particle].  Outputs are device arrays:  */

@@ -70,8 +70,16 @@ inline bool validate_params(const pimdb_params& p, std::string* why) {
     return true;
 }
 
-// Particles per tile of the column kernels (all P beads of tc particles).
-inline int column_tile(int P) { return std::max(1, std::min(64, 2048 / std::max(1, P))); }
+// Particles per tile of the column kernels (all P beads of tc particles):
+// about one output per thread of a 256-thread workgroup.
+inline int column_tile(int P) { return std::max(1, std::min(64, 256 / std::max(1, P))); }
+
+// Transposes of the normal-mode matrices for coalesced reads:
+// CTr[j][k] = C[k][j], CTT[k][b] = CT[b][k].
+inline void transpose_square(int P, const double* A, double* At) {
+    for (int r = 0; r < P; ++r)
+        for (int c = 0; c < P; ++c) At[(size_t)c * P + r] = A[(size_t)r * P + c];
+}
 
 // Cartesian <-> normal-mode matrices exactly as NormalModes builds them
 // (src/normal_modes.cpp:47-81): C[k][j] is row k as rank k computes it,
@@ -140,9 +148,7 @@ inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     return fa;
 }
 
-inline size_t rec_lds_bytes(int N) {
-    return sizeof(double) * (3 * (size_t)(N + 1) + 6 * REC_B + 2 * (size_t)REC_B * REC_B);
-}
+inline size_t rec_lds_bytes(int N) { return sizeof(double) * rec_shared_doubles(N); }
 
 inline size_t device_lds_limit() {
     static size_t lim = 0;
@@ -157,12 +163,12 @@ inline size_t device_lds_limit() {
 }
 
 inline void launch_cycle_energies(int D, int N, const double* x1, const double* xP, double k, int pbc, double L,
-                                  double* E, hipStream_t s) {
+                                  double* E, double* ET, hipStream_t s) {
     dim3 grid((N + 3) / 4), block(256);
     switch (D) {
-    case 1: hipLaunchKernelGGL(k_cycle_energies<1>, grid, block, 0, s, N, x1, xP, k, pbc, L, E); break;
-    case 2: hipLaunchKernelGGL(k_cycle_energies<2>, grid, block, 0, s, N, x1, xP, k, pbc, L, E); break;
-    default: hipLaunchKernelGGL(k_cycle_energies<3>, grid, block, 0, s, N, x1, xP, k, pbc, L, E); break;
+    case 1: hipLaunchKernelGGL(k_cycle_energies<1>, grid, block, 0, s, N, x1, xP, k, pbc, L, E, ET); break;
+    case 2: hipLaunchKernelGGL(k_cycle_energies<2>, grid, block, 0, s, N, x1, xP, k, pbc, L, E, ET); break;
+    default: hipLaunchKernelGGL(k_cycle_energies<3>, grid, block, 0, s, N, x1, xP, k, pbc, L, E, ET); break;
     }
 }
 
@@ -182,10 +188,10 @@ inline void launch_bead_energies_d(const pimdb_params& p, const Derived& d, cons
     dim3 grid((p.natoms + FORCE_TILE - 1) / FORCE_TILE, p.nbeads);
     const size_t lds = sizeof(double) * ((size_t)D * p.natoms + 3 * FORCE_WAVES);
     switch (p.int_kind) {
-    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_energies<D, POT_HARMONIC>), grid, dim3(256), lds, s, fa, x, part); break;
-    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_energies<D, POT_DIPOLE>), grid, dim3(256), lds, s, fa, x, part); break;
-    case POT_AZIZ: hipLaunchKernelGGL((k_bead_energies<D, POT_AZIZ>), grid, dim3(256), lds, s, fa, x, part); break;
-    default: hipLaunchKernelGGL((k_bead_energies<D, POT_FREE>), grid, dim3(256), lds, s, fa, x, part); break;
+    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_energies<D, POT_HARMONIC>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
+    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_energies<D, POT_DIPOLE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
+    case POT_AZIZ: hipLaunchKernelGGL((k_bead_energies<D, POT_AZIZ>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
+    default: hipLaunchKernelGGL((k_bead_energies<D, POT_FREE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
     }
 }
 

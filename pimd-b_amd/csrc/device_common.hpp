@@ -43,6 +43,23 @@ __device__ __forceinline__ double disp(double a, double b, bool pbc, double L) {
     return pbc ? min_image(d, L) : d;
 }
 
+// Same image as min_image(dx, L) without a division on the common path:
+// floor(dx * (1/L) + 1/2) equals floor(dx / L + 1/2) unless the argument lies
+// within a few ulp of an integer, which the guard sends to the exact path.
+__device__ __forceinline__ double min_image_fast(double dx, double L, double invL) {
+#pragma clang fp contract(off)
+    const double q = dx * invL + 0.5;
+    double fl = floor(q);
+    const double fr = q - fl;
+    if (fr < 1e-12 || fr > 1.0 - 1e-12) fl = floor(dx / L + 0.5);
+    return dx - L * fl;
+}
+__device__ __forceinline__ double disp_fast(double a, double b, bool pbc, double L, double invL) {
+#pragma clang fp contract(off)
+    double d = b - a;
+    return pbc ? min_image_fast(d, L, invL) : d;
+}
+
 // Squared norm, summed over axes in order without contraction.
 template <int D>
 __device__ __forceinline__ double norm2(const double (&d)[D]) {
@@ -90,6 +107,48 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
             g = (Aziz::epsilon / Aziz::rm) * (T1 + T2 + T3);
         }
         return g / norm;
+    } else {
+        return 0.0;
+    }
+}
+
+// g(r) as above from |d| and 1/|d| with products in place of the
+// reference's divisions (norm/rm, 1/r, g/norm): at most a few ulp apart.
+template <int KIND>
+__device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_norm, double kpair, double strength) {
+    if constexpr (KIND == POT_HARMONIC) {
+        return kpair;
+    } else if constexpr (KIND == POT_DIPOLE) {
+        const double i2 = inv_norm * inv_norm;
+        return -3.0 * strength * (i2 * i2 * inv_norm);
+    } else if constexpr (KIND == POT_AZIZ) {
+        constexpr double inv_rm = 1.0 / Aziz::rm;
+        const double r = norm * inv_rm;
+        const double T1 = -Aziz::A * Aziz::alpha * exp(-Aziz::alpha * r);
+        double g;
+        if (r > HARD_CORE_EPS && r < 0.01) {
+            g = T1 * (Aziz::epsilon / Aziz::rm);
+        } else {
+            const double ix = Aziz::rm * inv_norm;
+            const double ix2 = ix * ix;
+            const double ix6 = ix2 * ix2 * ix2;
+            const double ix7 = ix6 * ix;
+            const double ix8 = ix6 * ix2;
+            const double ix9 = ix8 * ix;
+            const double ix10 = ix8 * ix2;
+            const double ix11 = ix10 * ix;
+            double F = 1.0, dF = 0.0;
+            if (r < Aziz::D) {
+                const double t = Aziz::D * ix - 1.0;
+                const double ex = exp(-t * t);
+                F = ex;
+                dF = 2.0 * Aziz::D * ix * ix * t * ex;
+            }
+            const double T2 = (6.0 * Aziz::C6 * ix7 + 8.0 * Aziz::C8 * ix9 + 10.0 * Aziz::C10 * ix11) * F;
+            const double T3 = -(Aziz::C6 * ix6 + Aziz::C8 * ix8 + Aziz::C10 * ix10) * dF;
+            g = (Aziz::epsilon / Aziz::rm) * (T1 + T2 + T3);
+        }
+        return g * inv_norm;
     } else {
         return 0.0;
     }

@@ -32,7 +32,12 @@
 
 namespace pimdb {
 
-constexpr int REC_B = 32;   // targets per block (one wave does the sequential parts)
+constexpr int REC_B = 64;   // targets per block: one per lane of the sequential wave
+constexpr int REC_MAXWAVES = 16;
+constexpr int REC_UNROLL = 8;  // independent source loads in flight per lane
+
+// Row u of the row-major copy ET starts at u N - u(u-1)/2 and holds v = u..N-1.
+__host__ __device__ __forceinline__ size_t et_row(int u, int N) { return (size_t)u * N - (size_t)u * (u - 1) / 2; }
 
 // Cycle energies of column v, one wave per column.  ecyc(u,v) =
 // k/2 (sum_{w=u}^{v-1} |x1[w+1]-xP[w]|^2 + |xP[v]-x1[u]|^2): the closed form of
@@ -42,7 +47,7 @@ constexpr int REC_B = 32;   // targets per block (one wave does the sequential p
 template <int D>
 __device__ __forceinline__ void cycle_energy_column(int v, int N, int lane, const double* __restrict__ x1,
                                                     const double* __restrict__ xP, double k, bool pbc,
-                                                    double L, double* __restrict__ E) {
+                                                    double L, double* __restrict__ E, double* __restrict__ ET) {
     double xpv[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) xpv[a] = xP[a * N + v];
@@ -70,175 +75,201 @@ __device__ __forceinline__ void cycle_energy_column(int v, int N, int lane, cons
             const double o = __shfl_up(s, off, WAVE);
             if (lane >= off) s += o;
         }
-        if (u >= 0) E[col + u] = 0.5 * k * ((carry + s) + c);
+        if (u >= 0) {
+            const double e = 0.5 * k * ((carry + s) + c);
+            E[col + u] = e;
+            if (ET) ET[et_row(u, N) + (v - u)] = e;
+        }
         carry += __shfl(s, WAVE - 1, WAVE);
     }
 }
 
-// x1, xP: beads 1 and P, SoA [a][i].
+// x1, xP: beads 1 and P, SoA [a][i].  ET may be null.
 template <int D>
 static __global__ void __launch_bounds__(256) k_cycle_energies(int N, const double* __restrict__ x1,
                                                         const double* __restrict__ xP, double k, int pbc, double L,
-                                                        double* __restrict__ E) {
+                                                        double* __restrict__ E, double* __restrict__ ET) {
     const int v = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (v >= N) return;
-    cycle_energy_column<D>(v, N, threadIdx.x % WAVE, x1, xP, k, pbc != 0, L, E);
+    cycle_energy_column<D>(v, N, threadIdx.x % WAVE, x1, xP, k, pbc != 0, L, E, ET);
 }
 
-// Shared-memory workspace of one workgroup running both recursions.
+// Shared-memory workspace of one workgroup running one recursion.
 struct RecShared {
-    double* U[2];      // [N+1] each: U of forward / backward
+    double* U;         // [N+1]
     double* lg;        // [N+1]  ln(i)
-    double* R[2];      // [REC_B]
-    double* z[2];      // [REC_B]
-    double* acc[2];    // [REC_B]  phase partials
-    double* a[2];      // [REC_B*REC_B] in-block coefficients a[j*REC_B + t]
+    double* R;         // [REC_B] shifts of the block's targets
+    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min / sum
+    double* a;         // [REC_B*REC_B] in-block weights, then coefficients, a[j*REC_B + t]
 };
 
-__device__ __forceinline__ size_t rec_shared_doubles(int N) {
-    return 3 * (size_t)(N + 1) + 6 * REC_B + 2 * (size_t)REC_B * REC_B;
+__host__ __device__ __forceinline__ size_t rec_shared_doubles(int N) {
+    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
 }
 
 __device__ __forceinline__ RecShared rec_carve(double* base, int N) {
     RecShared s;
     double* p = base;
-    s.U[0] = p; p += N + 1;
-    s.U[1] = p; p += N + 1;
+    s.U = p; p += N + 1;
     s.lg = p; p += N + 1;
-    for (int d = 0; d < 2; ++d) { s.R[d] = p; p += REC_B; }
-    for (int d = 0; d < 2; ++d) { s.z[d] = p; p += REC_B; }
-    for (int d = 0; d < 2; ++d) { s.acc[d] = p; p += REC_B; }
-    for (int d = 0; d < 2; ++d) { s.a[d] = p; p += (size_t)REC_B * REC_B; }
+    s.R = p; p += REC_B;
+    s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
+    s.a = p;
     return s;
 }
 
-// w(n,q) of direction dir (see file header).
-__device__ __forceinline__ double rec_w(int dir, int N, const double* __restrict__ E, double bt,
+// w(n,q) of direction dir (see file header).  The forward direction reads
+// the row-major copy ET (row q, contiguous in n), the backward one reads E
+// (column N-1-q, contiguous in n): consecutive targets hit consecutive
+// addresses in both.
+template <int DIR>
+__device__ __forceinline__ double rec_w(int N, const double* __restrict__ E, const double* __restrict__ ET, double bt,
                                         const double* lg, int n, int q) {
-    if (dir == 0) return bt * E[(size_t)(n - 1) * n / 2 + q] + lg[n];
-    const int p = N - 1 - q;
-    return bt * E[(size_t)p * (p + 1) / 2 + (N - n)] + lg[N - q];
+    if constexpr (DIR == 0) {
+        return bt * ET[et_row(q, N) + (n - 1 - q)] + lg[n];
+    } else {
+        const int p = N - 1 - q;
+        return bt * E[(size_t)p * (p + 1) / 2 + (N - n)] + lg[N - q];
+    }
 }
 
-// Both recursions for one ring, executed by the whole workgroup (any
-// multiple of 128 threads).  Writes V[0..N], Vb[0..N] and sets *status to 1
-// when a potential is not finite (the reference throws overflow_error,
-// quadratic_bosonic_exchange.cpp:92-97, 119-124).
-__device__ void exchange_recursions(int N, double bt, const double* __restrict__ E, double* __restrict__ V,
-                                    double* __restrict__ Vb, int* __restrict__ status, RecShared sh) {
+// One recursion (DIR 0: forward V, DIR 1: backward Vb) for one ring, run by
+// the whole workgroup (64..1024 threads).  Lane t of every wave serves target
+// n0+t of the current block; the waves split the sources of earlier blocks
+// and keep REC_UNROLL loads in flight.  Writes V[0..N] and Vb[0] (DIR 0) or
+// Vb[1..N] (DIR 1); sets *status to 1 when a potential is not finite (the
+// reference throws overflow_error, quadratic_bosonic_exchange.cpp:92-97,
+// 119-124).
+template <int DIR>
+__device__ void exchange_recursion(int N, double bt, const double* __restrict__ E, const double* __restrict__ ET,
+                                   double* __restrict__ V, double* __restrict__ Vb, int* __restrict__ status,
+                                   RecShared sh) {
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int lane = tid % WAVE, wave = tid / WAVE, nwave = nthr / WAVE;
-    const int ntarget[2] = {N, N - 1};
+    const int lane = tid % WAVE, wave = tid / WAVE;
+    const int nwave = min(nthr / WAVE, REC_MAXWAVES);
+    const int ntarget = DIR == 0 ? N : N - 1;
     for (int i = tid; i <= N; i += nthr) {
         sh.lg[i] = (i > 0) ? log((double)i) : 0.0;
-        sh.U[0][i] = 0.0;
-        sh.U[1][i] = 0.0;
+        sh.U[i] = 0.0;
     }
     __syncthreads();
-    for (int n0 = 1; n0 <= N; n0 += REC_B) {
-        int nb[2];
-        for (int d = 0; d < 2; ++d) nb[d] = max(0, min(REC_B, ntarget[d] - n0 + 1));
-        // (0) in-block weights w(n0+t, n0+j), j < t, staged in LDS as a[j][t]
-        for (int idx = tid; idx < 2 * REC_B * REC_B; idx += nthr) {
-            const int d = idx / (REC_B * REC_B);
-            const int j = (idx / REC_B) % REC_B, t = idx % REC_B;
-            sh.a[d][j * REC_B + t] = (t < nb[d] && j < t) ? rec_w(d, N, E, bt, sh.lg, n0 + t, n0 + j) : 0.0;
+    for (int n0 = 1; n0 <= ntarget; n0 += REC_B) {
+        const int nb = min(REC_B, ntarget - n0 + 1);
+        // (0) in-block weights w(n0+t, n0+j), j < t, staged as a[j][t]
+        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
+            const int j = idx / REC_B, t = idx % REC_B;
+            sh.a[idx] = (t < nb && j < t) ? rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + t, n0 + j) : 0.0;
         }
-        // (1a) min over sources of earlier blocks, one wave per (dir, target)
-        for (int task = wave; task < 2 * REC_B; task += nwave) {
-            const int d = task / REC_B, t = task % REC_B;
-            if (t >= nb[d]) continue;
-            const int n = n0 + t;
+        // (1a) per-wave partial min over this wave's share of earlier sources
+        const int chunk = (n0 + nwave - 1) / nwave;
+        const int qlo = min(n0, wave * chunk), qhi = min(n0, qlo + chunk);
+        const bool active = wave < nwave && lane < nb;
+        if (wave < nwave) {
             double m = INFINITY;
-            for (int q = lane; q < n0; q += WAVE) m = fmin(m, rec_w(d, N, E, bt, sh.lg, n, q) + sh.U[d][q]);
-            m = wave_min(m);
-            if (lane == 0) sh.acc[d][t] = m;
+            if (active) {
+                int q = qlo;
+                for (; q + REC_UNROLL <= qhi; q += REC_UNROLL) {
+                    double w[REC_UNROLL];
+#pragma unroll
+                    for (int r = 0; r < REC_UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+#pragma unroll
+                    for (int r = 0; r < REC_UNROLL; ++r) m = fmin(m, w[r] + sh.U[q + r]);
+                }
+                for (; q < qhi; ++q) m = fmin(m, rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q]);
+            }
+            sh.part[wave * REC_B + lane] = m;
         }
         __syncthreads();
-        // (1b) in-block min-plus: wave 0 forward, wave 1 backward
-        if (wave < 2) {
-            const int d = wave;
+        // (1b) in-block min-plus on wave 0
+        if (wave == 0) {
             const int t = lane;
-            const double* wb = sh.a[d];
-            double r = (t < nb[d]) ? sh.acc[d][t] : INFINITY;
-            double wnext = wb[t];
-            for (int j = 0; j + 1 < nb[d]; ++j) {
+            double r = INFINITY;
+            for (int w = 0; w < nwave; ++w) r = fmin(r, sh.part[w * REC_B + t]);
+            if (t >= nb) r = INFINITY;
+            double wnext = sh.a[t];
+            for (int j = 0; j + 1 < nb; ++j) {
                 const double wj = wnext;
-                wnext = wb[(j + 1) * REC_B + t];
+                wnext = sh.a[(j + 1) * REC_B + t];
                 const double Rj = wave_bcast_uniform(r, j);
                 if (t > j) r = fmin(r, wj + Rj);
             }
-            if (t < nb[d]) sh.R[d][t] = r;
+            sh.R[t] = r;
         }
         __syncthreads();
-        // (2a) sum over sources of earlier blocks
-        for (int task = wave; task < 2 * REC_B; task += nwave) {
-            const int d = task / REC_B, t = task % REC_B;
-            if (t >= nb[d]) continue;
-            const int n = n0 + t;
-            const double Rn = sh.R[d][t];
+        // (2a) per-wave partial sums over earlier sources
+        if (wave < nwave) {
             double s = 0.0;
-            for (int q = lane; q < n0; q += WAVE) s += exp(-(rec_w(d, N, E, bt, sh.lg, n, q) + sh.U[d][q] - Rn));
-            s = wave_sum(s);
-            if (lane == 0) sh.z[d][t] = s;
+            if (active) {
+                const double Rn = sh.R[lane];
+                int q = qlo;
+                for (; q + REC_UNROLL <= qhi; q += REC_UNROLL) {
+                    double w[REC_UNROLL];
+#pragma unroll
+                    for (int r = 0; r < REC_UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+#pragma unroll
+                    for (int r = 0; r < REC_UNROLL; ++r) s += exp(-(w[r] + sh.U[q + r] - Rn));
+                }
+                for (; q < qhi; ++q) s += exp(-(rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q] - Rn));
+            }
+            sh.part[wave * REC_B + lane] = s;
         }
         // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)), j < t
-        for (int idx = tid; idx < 2 * REC_B * REC_B; idx += nthr) {
-            const int d = idx / (REC_B * REC_B);
-            const int j = (idx / REC_B) % REC_B, t = idx % REC_B;
-            if (t < nb[d] && j < t) {
-                double* ap = &sh.a[d][j * REC_B + t];
-                *ap = exp(-(*ap + sh.R[d][j] - sh.R[d][t]));
-            }
+        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
+            const int j = idx / REC_B, t = idx % REC_B;
+            if (t < nb && j < t) sh.a[idx] = exp(-(sh.a[idx] + sh.R[j] - sh.R[t]));
         }
         __syncthreads();
-        // (2c) in-block forward substitution
-        if (wave < 2) {
-            const int d = wave;
+        // (2c) in-block forward substitution on wave 0, then U = R - ln z
+        if (wave == 0) {
             const int t = lane;
-            const double* ab = sh.a[d];
-            double z = (t < nb[d]) ? sh.z[d][t] : 0.0;
-            double anext = ab[t];
-            for (int j = 0; j + 1 < nb[d]; ++j) {
+            double z = 0.0;
+            for (int w = 0; w < nwave; ++w) z += sh.part[w * REC_B + t];
+            double anext = sh.a[t];
+            for (int j = 0; j + 1 < nb; ++j) {
                 const double aj = anext;
-                anext = ab[(j + 1) * REC_B + t];
+                anext = sh.a[(j + 1) * REC_B + t];
                 const double zj = wave_bcast_uniform(z, j);
                 z = fma(aj, zj, z);
             }
-            if (t < nb[d]) {
-                const double u = sh.R[d][t] - log(z);
-                sh.U[d][n0 + t] = u;
+            if (t < nb) {
+                const double u = sh.R[t] - log(z);
+                sh.U[n0 + t] = u;
                 if (!isfinite(u)) atomicOr(status, 1);
             }
         }
         __syncthreads();
     }
     // outputs in the reference's units
-    for (int i = tid; i <= N; i += nthr) {
-        V[i] = sh.U[0][i] / bt;
-        if (i >= 1 && i < N) Vb[i] = sh.U[1][N - i] / bt;
-    }
-    if (tid == 0) {
-        Vb[N] = 0.0;
-        Vb[0] = sh.U[0][N] / bt;   // V_backwards[0] = V[N], :127
+    if constexpr (DIR == 0) {
+        for (int i = tid; i <= N; i += nthr) V[i] = sh.U[i] / bt;
+        if (tid == 0) Vb[0] = sh.U[N] / bt;   // V_backwards[0] = V[N], :127
+    } else {
+        for (int i = tid; i <= N; i += nthr) {
+            if (i >= 1 && i < N) Vb[i] = sh.U[N - i] / bt;
+        }
+        if (tid == 0) Vb[N] = 0.0;
     }
     __syncthreads();
 }
 
+// Block 0 runs the forward recursion, block 1 the backward one.
 static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, double bt, const double* __restrict__ E,
-                                                              double* __restrict__ V, double* __restrict__ Vb,
-                                                              int* __restrict__ status) {
+                                                                     const double* __restrict__ ET,
+                                                                     double* __restrict__ V, double* __restrict__ Vb,
+                                                                     int* __restrict__ status) {
     extern __shared__ double smem[];
-    exchange_recursions(N, bt, E, V, Vb, status, rec_carve(smem, N));
+    if (blockIdx.x == 0) exchange_recursion<0>(N, bt, E, ET, V, Vb, status, rec_carve(smem, N));
+    else exchange_recursion<1>(N, bt, E, ET, V, Vb, status, rec_carve(smem, N));
 }
 
 // Connection probability P[l][u] (quadratic_bosonic_exchange.cpp:142-157).
+// With ET given, ecyc(u,l) is read from the row-major copy.
 __device__ __forceinline__ double conn_prob(int N, double bt, const double* __restrict__ E,
-                                            const double* __restrict__ V, const double* __restrict__ Vb,
-                                            int l, int u) {
+                                            const double* __restrict__ ET, const double* __restrict__ V,
+                                            const double* __restrict__ Vb, int l, int u) {
     if (u == l + 1 && l < N - 1) return 1.0 - (exp(-bt * (V[l + 1] + Vb[l + 1] - V[N])));
     if (u <= l) {
-        const double e = E[(size_t)l * (l + 1) / 2 + u];
+        const double e = ET ? ET[et_row(u, N) + (l - u)] : E[(size_t)l * (l + 1) / 2 + u];
         return 1.0 / (l + 1) * exp(-bt * (V[u] + e + Vb[l + 1] - V[N]));
     }
     return 0.0;
@@ -251,7 +282,7 @@ static __global__ void __launch_bounds__(256) k_exchange_probabilities(int N, do
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)N * N) return;
     const int l = (int)(idx / N), u = (int)(idx % N);
-    prob[idx] = conn_prob(N, bt, E, V, Vb, l, u);
+    prob[idx] = conn_prob(N, bt, E, nullptr, V, Vb, l, u);
 }
 
 // Exterior spring force of particle l on bead 1 (which=0) or bead P
@@ -263,7 +294,8 @@ static __global__ void __launch_bounds__(256) k_exchange_probabilities(int N, do
 // alone, used by the autograd harness).
 template <int D>
 __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane, double bt, double k, bool pbc,
-                                               double L, const double* __restrict__ E, const double* __restrict__ V,
+                                               double L, const double* __restrict__ E, const double* __restrict__ ET,
+                                               const double* __restrict__ V,
                                                const double* __restrict__ Vb, const double* __restrict__ x1,
                                                const double* __restrict__ x2, const double* __restrict__ xPm1,
                                                const double* __restrict__ xP, bool grad_only, double (&F)[D]) {
@@ -278,7 +310,8 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
     const int jlo = which == 0 ? max(0, l - 1) : 0;
     const int jhi = which == 0 ? N - 1 : min(l + 1, N - 1);
     for (int j = jlo + lane; j <= jhi; j += WAVE) {
-        const double pr = which == 0 ? conn_prob(N, bt, E, V, Vb, j, l) : conn_prob(N, bt, E, V, Vb, l, j);
+        // bead 1 walks column l of P (row l of ET), bead P walks row l (column l of E)
+        const double pr = which == 0 ? conn_prob(N, bt, E, ET, V, Vb, j, l) : conn_prob(N, bt, E, nullptr, V, Vb, l, j);
 #pragma unroll
         for (int a = 0; a < D; ++a) S[a] += pr * disp(xs[a], other[a * N + j], pbc, L);
     }
@@ -297,7 +330,8 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
 // f = F_exchange + f_phys (simulation.cpp:363-373: spring + physical).
 template <int D>
 static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, double bt, double k, int pbc, double L,
-                                                         const double* __restrict__ E, const double* __restrict__ V,
+                                                         const double* __restrict__ E, const double* __restrict__ ET,
+                                                         const double* __restrict__ V,
                                                          const double* __restrict__ Vb, const double* __restrict__ x,
                                                          double* __restrict__ f, double* __restrict__ fex) {
     const int task = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -310,7 +344,7 @@ static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, do
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
     const double* xP = x + bs * (P - 1);
     double F[D];
-    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, V, Vb, x1, x2, xPm1, xP, false, F);
+    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, ET, V, Vb, x1, x2, xPm1, xP, false, F);
     if (lane == 0) {
         const size_t b = which == 0 ? 0 : (size_t)(P - 1);
 #pragma unroll
@@ -326,7 +360,8 @@ static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, do
 // given separately in SoA [a][i]; out[which][a][i].
 template <int D>
 static __global__ void __launch_bounds__(256) k_exchange_vb_grad(int N, double bt, double k, int pbc, double L,
-                                                          const double* __restrict__ E, const double* __restrict__ V,
+                                                          const double* __restrict__ E, const double* __restrict__ ET,
+                                                          const double* __restrict__ V,
                                                           const double* __restrict__ Vb, const double* __restrict__ x1,
                                                           const double* __restrict__ xP, double* __restrict__ out) {
     const int task = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -334,7 +369,7 @@ static __global__ void __launch_bounds__(256) k_exchange_vb_grad(int N, double b
     const int which = task / N, l = task % N;
     const int lane = threadIdx.x % WAVE;
     double F[D];
-    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, V, Vb, x1, x1, xP, xP, true, F);
+    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, ET, V, Vb, x1, x1, xP, xP, true, F);
     if (lane == 0) {
 #pragma unroll
         for (int a = 0; a < D; ++a) out[((size_t)which * D + a) * N + l] = F[a];

@@ -6,8 +6,9 @@
 // src/potentials/{harmonic,dipole,aziz}.cpp.
 //
 // One workgroup handles 64 particles i of one bead (time slice) b: the bead's
-// N x D positions are staged in LDS, each of the 4 waves runs a quarter of the
-// partners j, and the partial forces are reduced in LDS.  Every ordered pair
+// N x D positions are staged in LDS, each of the 16 waves runs a sixteenth of
+// the partners j (four waves per SIMD hide the FP64 latency chains), and the
+// partial forces are reduced in LDS.  Every ordered pair
 // (i,j) evaluates gradV(r_i - r_j); for central potentials this is bitwise
 // the negation of gradV(r_j - r_i), so the result equals the reference's
 // Newton's-third-law loop up to summation order.
@@ -27,7 +28,7 @@ struct ForceArgs {
 };
 
 constexpr int FORCE_TILE = 64;
-constexpr int FORCE_WAVES = 4;
+constexpr int FORCE_WAVES = 16;
 
 template <int D, int KIND>
 static __global__ void __launch_bounds__(FORCE_TILE * FORCE_WAVES)
@@ -55,15 +56,16 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
             for (int a = 0; a < D; ++a) xi[a] = sx[a * N + i];
             const int chunk = (N + FORCE_WAVES - 1) / FORCE_WAVES;
             const int jlo = wave * chunk, jhi = min(N, jlo + chunk);
+            const double invL = 1.0 / fa.L;
             for (int j = jlo; j < jhi; ++j) {
                 if (j == i) continue;
                 double d[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) d[a] = disp(sx[a * N + j], xi[a], pbc, fa.L);   // r_i - r_j
+                for (int a = 0; a < D; ++a) d[a] = disp_fast(sx[a * N + j], xi[a], pbc, fa.L, invL);   // r_i - r_j
                 const double r2 = norm2<D>(d);
                 const double norm = sqrt(r2);
                 if (norm < fa.cutoff || fa.cutoff < 0.0) {
-                    const double g = pair_grad_factor<KIND>(norm, fa.int_k, fa.strength);
+                    const double g = pair_grad_factor_fast<KIND>(norm, 1.0 / norm, fa.int_k, fa.strength);
 #pragma unroll
                     for (int a = 0; a < D; ++a) acc[a] += -(g * d[a]);
                 }

@@ -25,8 +25,8 @@ struct MomentaArgs {
     double c1, c2;              // Langevin friction / noise (langevin.cpp:10-13)
     int thermostat;             // 0 none, 1 langevin
     int nm;                     // normal-mode coupling (thermostat_coupling.cpp:19-47)
-    const double* C;            // [P][P] cart->nm, rows as built per rank
-    const double* CT;           // [P][P] nm->cart
+    const double* Ct;           // cart->nm matrix transposed: Ct[j*P + k] = C[k][j]
+    const double* CTt;          // nm->cart matrix transposed: CTt[k*P + b] = CT[b][k]
     const double* noise;        // host noise [b][a][i] for this O-step, or null (Philox)
     const unsigned long long* step_ctr;
     uint32_t ostep, seed, replica;
@@ -36,10 +36,19 @@ struct MomentaArgs {
     const double* kick_f;       // forces for a half kick first, or null
 };
 
-__device__ __forceinline__ double com_value(const double* part, int npart, int D, int a, int N, int P) {
-    double s = 0.0;
-    for (int t = 0; t < npart; ++t) s += part[(size_t)t * D + a];
-    return s / (double)(N * P);
+// Centre-of-mass momentum per axis from the partial sums [npart][D], reduced
+// once per workgroup into scm[0..D-1] (wave a reduces axis a); every
+// workgroup runs the same reduction tree, so all see the same values.  Must
+// be called by all threads of the workgroup.
+__device__ __forceinline__ void block_com(const double* part, int npart, int D, int N, int P, double* scm) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (wave < D) {
+        double s = 0.0;
+        for (int t = lane; t < npart; t += WAVE) s += part[(size_t)t * D + wave];
+        s = wave_sum(s);
+        if (lane == 0) scm[wave] = s / (double)(N * P);
+    }
+    __syncthreads();
 }
 
 __device__ __forceinline__ double noise_at(const MomentaArgs& m, int b, int a, int i) {
@@ -62,7 +71,9 @@ static __global__ void __launch_bounds__(256) k_momenta(MomentaArgs m, double* _
     const int i0 = blockIdx.x * TC;
     const int tid = threadIdx.x;
     const int ncol = min(TC, N - i0);
-    const double cm = m.com_in ? com_value(m.com_in, m.ncom_in, D, a, N, P) : 0.0;
+    __shared__ double scm[4];
+    if (m.com_in) block_com(m.com_in, m.ncom_in, D, N, P, scm);
+    const double cm = m.com_in ? scm[a] : 0.0;
     for (int e = tid; e < P * TC; e += blockDim.x) {
         const int b = e / TC, c = e % TC;
         double v = 0.0;
@@ -86,16 +97,14 @@ static __global__ void __launch_bounds__(256) k_momenta(MomentaArgs m, double* _
             for (int e = tid; e < P * TC; e += blockDim.x) {
                 const int k = e / TC, c = e % TC;
                 double s = 0.0;
-                const double* row = m.C + (size_t)k * P;
-                for (int j = 0; j < P; ++j) s += row[j] * pc[j * TC + c];
+                for (int j = 0; j < P; ++j) s += m.Ct[(size_t)j * P + k] * pc[j * TC + c];
                 pn[e] = (c < ncol) ? m.c1 * s + m.c2 * noise_at(m, k, a, i0 + c) : 0.0;
             }
             __syncthreads();
             for (int e = tid; e < P * TC; e += blockDim.x) {
                 const int b = e / TC, c = e % TC;
                 double s = 0.0;
-                const double* row = m.CT + (size_t)b * P;
-                for (int k = 0; k < P; ++k) s += row[k] * pn[k * TC + c];
+                for (int k = 0; k < P; ++k) s += m.CTt[(size_t)k * P + b] * pn[k * TC + c];
                 pc[e] = s;
             }
         }
@@ -134,11 +143,13 @@ static __global__ void __launch_bounds__(256) k_kick_drift(DriftArgs d, double* 
                                                     const double* __restrict__ f) {
     const size_t tot = (size_t)d.P * d.D * d.N;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double scm[4];
+    if (d.com_in) block_com(d.com_in, d.ncom_in, d.D, d.N, d.P, scm);
     if (e == 0 && d.step_ctr) *d.step_ctr += 1;
     if (e >= tot) return;
     const int a = (int)((e / d.N) % d.D);
     double pv = p[e];
-    if (d.com_in) pv -= com_value(d.com_in, d.ncom_in, d.D, a, d.N, d.P);
+    if (d.com_in) pv -= scm[a];
     pv += d.half_dt * f[e];
     p[e] = pv;
     x[e] += d.dt * pv / d.mass;
@@ -149,16 +160,18 @@ static __global__ void __launch_bounds__(256) k_com_subtract(int N, int P, int D
                                                       int ncom_in, double* __restrict__ p) {
     const size_t tot = (size_t)P * D * N;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double scm[4];
+    block_com(com_in, ncom_in, D, N, P, scm);
     if (e >= tot) return;
     const int a = (int)((e / N) % D);
-    p[e] -= com_value(com_in, ncom_in, D, a, N, P);
+    p[e] -= scm[a];
 }
 
 struct NmPropArgs {
     int N, P, D, tc;
     double half_dt, dt, mass;
-    const double* C;
-    const double* CT;
+    const double* Ct;       // C transposed, as in MomentaArgs
+    const double* CTt;      // CT transposed
     const double* rot;      // [P][4]: cos(w dt), sin(w dt), m w, w
     const double* com_in;
     int ncom_in;
@@ -181,7 +194,9 @@ static __global__ void __launch_bounds__(256) k_nm_propagate(NmPropArgs m, doubl
     const int tid = threadIdx.x;
     const int ncol = min(TC, N - i0);
     if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && m.step_ctr) *m.step_ctr += 1;
-    const double cm = m.com_in ? com_value(m.com_in, m.ncom_in, D, a, N, P) : 0.0;
+    __shared__ double scm[4];
+    if (m.com_in) block_com(m.com_in, m.ncom_in, D, N, P, scm);
+    const double cm = m.com_in ? scm[a] : 0.0;
     for (int e = tid; e < P * TC; e += blockDim.x) {
         const int b = e / TC, c = e % TC;
         double xv = 0.0, pv = 0.0;
@@ -198,10 +213,12 @@ static __global__ void __launch_bounds__(256) k_nm_propagate(NmPropArgs m, doubl
     __syncthreads();
     for (int e = tid; e < P * TC; e += blockDim.x) {
         const int k = e / TC, c = e % TC;
-        const double* row = m.C + (size_t)k * P;
         double q = 0.0, pp = 0.0;
-        for (int j = 0; j < P; ++j) q += row[j] * xc[j * TC + c];
-        for (int j = 0; j < P; ++j) pp += row[j] * pc[j * TC + c];
+        for (int j = 0; j < P; ++j) {
+            const double cj = m.Ct[(size_t)j * P + k];
+            q += cj * xc[j * TC + c];
+            pp += cj * pc[j * TC + c];
+        }
         const double cs = m.rot[4 * k], sn = m.rot[4 * k + 1], mw = m.rot[4 * k + 2], w = m.rot[4 * k + 3];
         if (w == 0) {
             qn[e] = q + m.dt / m.mass * pp;
@@ -215,10 +232,12 @@ static __global__ void __launch_bounds__(256) k_nm_propagate(NmPropArgs m, doubl
     for (int e = tid; e < P * TC; e += blockDim.x) {
         const int b = e / TC, c = e % TC;
         if (c >= ncol) continue;
-        const double* row = m.CT + (size_t)b * P;
         double xv = 0.0, pv = 0.0;
-        for (int k = 0; k < P; ++k) xv += row[k] * qn[k * TC + c];
-        for (int k = 0; k < P; ++k) pv += row[k] * pn[k * TC + c];
+        for (int k = 0; k < P; ++k) {
+            const double ck = m.CTt[(size_t)k * P + b];
+            xv += ck * qn[k * TC + c];
+            pv += ck * pn[k * TC + c];
+        }
         const size_t g = ((size_t)b * D + a) * N + i0 + c;
         x[g] = xv;
         p[g] = pv;

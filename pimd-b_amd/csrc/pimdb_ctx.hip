@@ -35,7 +35,7 @@ struct pimdb_ctx {
     hipStream_t s0 = nullptr, s1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, t0 = nullptr, t1 = nullptr;
     double *x = nullptr, *p = nullptr, *f = nullptr, *fext = nullptr;
-    double *E = nullptr, *V = nullptr, *Vb = nullptr, *prob = nullptr, *fex = nullptr;
+    double *E = nullptr, *ET = nullptr, *V = nullptr, *Vb = nullptr, *prob = nullptr, *fex = nullptr;
     double *C = nullptr, *CT = nullptr, *rot = nullptr, *com0 = nullptr, *com1 = nullptr;
     double *noise = nullptr, *obs = nullptr;
     size_t noise_cap = 0;
@@ -68,10 +68,10 @@ static void launch_bead_forces_d(pimdb_ctx* ctx, hipStream_t s, double* f, doubl
     dim3 grid((ctx->N + FORCE_TILE - 1) / FORCE_TILE, ctx->P);
     const size_t lds = sizeof(double) * ((size_t)D * ctx->N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE);
     switch (ctx->prm.int_kind) {
-    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_forces<D, POT_HARMONIC>), grid, dim3(256), lds, s, fa, ctx->x, f, fext); break;
-    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_forces<D, POT_DIPOLE>), grid, dim3(256), lds, s, fa, ctx->x, f, fext); break;
-    case POT_AZIZ: hipLaunchKernelGGL((k_bead_forces<D, POT_AZIZ>), grid, dim3(256), lds, s, fa, ctx->x, f, fext); break;
-    default: hipLaunchKernelGGL((k_bead_forces<D, POT_FREE>), grid, dim3(256), lds, s, fa, ctx->x, f, fext); break;
+    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_forces<D, POT_HARMONIC>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
+    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_forces<D, POT_DIPOLE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
+    case POT_AZIZ: hipLaunchKernelGGL((k_bead_forces<D, POT_AZIZ>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
+    default: hipLaunchKernelGGL((k_bead_forces<D, POT_FREE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
     }
 }
 
@@ -88,16 +88,16 @@ static void launch_exchange_forward(pimdb_ctx* ctx, hipStream_t s) {
     const int N = ctx->N, P = ctx->P;
     const double* x1 = ctx->x;
     const double* xP = ctx->x + (size_t)(P - 1) * ctx->D * N;
-    launch_cycle_energies(ctx->D, N, x1, xP, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, s);
-    hipLaunchKernelGGL(k_exchange_recursions, dim3(1), dim3(1024), rec_lds_bytes(N), s, N, ctx->dv.bt, ctx->E,
-                       ctx->V, ctx->Vb, ctx->status);
+    launch_cycle_energies(ctx->D, N, x1, xP, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, ctx->ET, s);
+    hipLaunchKernelGGL(k_exchange_recursions, dim3(2), dim3(1024), rec_lds_bytes(N), s, N, ctx->dv.bt, ctx->E,
+                       ctx->ET, ctx->V, ctx->Vb, ctx->status);
 }
 
 template <int D>
 static void launch_exchange_forces_d(pimdb_ctx* ctx, hipStream_t s, double* f, double* fex) {
     const int N = ctx->N;
     hipLaunchKernelGGL((k_exchange_forces<D>), dim3((2 * N + 3) / 4), dim3(256), 0, s, N, ctx->P, ctx->dv.bt,
-                       ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, ctx->V, ctx->Vb, ctx->x, f, fex);
+                       ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->x, f, fex);
 }
 
 static void launch_exchange_forces(pimdb_ctx* ctx, hipStream_t s, double* f, double* fex) {
@@ -132,7 +132,7 @@ static MomentaArgs momenta_args(pimdb_ctx* ctx, int ostep, const double* noise) 
     m.c1 = ctx->dv.c1; m.c2 = ctx->dv.c2;
     m.thermostat = ctx->prm.thermostat;
     m.nm = ctx->prm.nmthermostat;
-    m.C = ctx->C; m.CT = ctx->CT;
+    m.Ct = ctx->C; m.CTt = ctx->CT;
     m.noise = noise;
     m.step_ctr = ctx->ctr;
     m.ostep = (uint32_t)ostep;
@@ -161,7 +161,7 @@ static int enqueue_step(pimdb_ctx* ctx, const double* noise0, const double* nois
         NmPropArgs a{};
         a.N = N; a.P = P; a.D = D; a.tc = ctx->tc;
         a.half_dt = 0.5 * ctx->prm.dt; a.dt = ctx->prm.dt; a.mass = ctx->prm.mass;
-        a.C = ctx->C; a.CT = ctx->CT; a.rot = ctx->rot;
+        a.Ct = ctx->C; a.CTt = ctx->CT; a.rot = ctx->rot;
         a.com_in = fixcom ? ctx->com0 : nullptr; a.ncom_in = ctx->ntiles;
         a.fext = ctx->fext; a.step_ctr = ctx->ctr;
         hipLaunchKernelGGL(k_nm_propagate, tgrid, dim3(256), sizeof(double) * 4 * (size_t)P * ctx->tc, ctx->s0, a,
@@ -257,6 +257,7 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     alloc(&ctx->f, sizeof(double) * ctx->nbd);
     alloc(&ctx->fext, sizeof(double) * ctx->nbd);
     alloc(&ctx->E, sizeof(double) * ((size_t)N * (N + 1) / 2));
+    alloc(&ctx->ET, sizeof(double) * ((size_t)N * (N + 1) / 2));
     alloc(&ctx->V, sizeof(double) * (N + 1));
     alloc(&ctx->Vb, sizeof(double) * (N + 1));
     alloc(&ctx->prob, sizeof(double) * (size_t)N * N);
@@ -276,8 +277,11 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
                 hipEventCreate(&ctx->t0) != hipSuccess || hipEventCreate(&ctx->t1) != hipSuccess))
         rc = ctx->fail(PIMDB_E_HIP, "stream/event creation failed");
     if (!rc) {
-        std::vector<double> Cm((size_t)P * P), CTm((size_t)P * P), rot(4 * (size_t)P);
-        nm_matrices(P, Cm.data(), CTm.data());
+        std::vector<double> C0((size_t)P * P), CT0((size_t)P * P), Cm((size_t)P * P), CTm((size_t)P * P),
+            rot(4 * (size_t)P);
+        nm_matrices(P, C0.data(), CT0.data());
+        transpose_square(P, C0.data(), Cm.data());    // device holds C^T: [j][k]
+        transpose_square(P, CT0.data(), CTm.data());  // and CT^T: [k][b]
         nm_rotation(*prm, ctx->dv, rot.data());
         if (hipMemcpy(ctx->C, Cm.data(), sizeof(double) * Cm.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ctx->CT, CTm.data(), sizeof(double) * CTm.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -286,7 +290,7 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     }
     if (!rc && ctx->dv.bosonic && rec_lds_bytes(N) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the exchange workgroup");
-    if (!rc && sizeof(double) * ((size_t)D * N + 3 * D * FORCE_TILE) > device_lds_limit())
+    if (!rc && sizeof(double) * ((size_t)D * N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the force workgroup");
     if (rc) {
         set_create_error(ctx->err);
@@ -303,7 +307,7 @@ void pimdb_ctx_destroy(pimdb_ctx* ctx) {
     if (ctx->s1) (void)hipStreamSynchronize(ctx->s1);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    double* bufs[] = {ctx->x, ctx->p, ctx->f, ctx->fext, ctx->E, ctx->V, ctx->Vb, ctx->prob, ctx->fex,
+    double* bufs[] = {ctx->x, ctx->p, ctx->f, ctx->fext, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->prob, ctx->fex,
                       ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
@@ -485,46 +489,70 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
     return PIMDB_OK;
 }
 
+}  // extern "C"
+
+// Device time per repetition of `body`, from `reps` repetitions captured in
+// one hipGraph and replayed on the context stream between two events (so
+// host launch latency is not counted).
+template <class F>
+static int time_captured(pimdb_ctx* ctx, int reps, F body, double* ms_out) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    CK(hipStreamSynchronize(ctx->s0));
+    CK(hipStreamBeginCapture(ctx->s0, hipStreamCaptureModeThreadLocal));
+    int rc = PIMDB_OK;
+    for (int r = 0; r < reps && !rc; ++r) rc = body();
+    hipError_t e = hipStreamEndCapture(ctx->s0, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return ctx->fail(PIMDB_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return ctx->fail(PIMDB_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    float ms = 0.0f;
+    (void)hipGraphLaunch(ge, ctx->s0);   // warm
+    (void)hipEventRecord(ctx->t0, ctx->s0);
+    (void)hipGraphLaunch(ge, ctx->s0);
+    (void)hipEventRecord(ctx->t1, ctx->s0);
+    e = hipEventSynchronize(ctx->t1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return ctx->fail(PIMDB_E_HIP, std::string("graph timing: ") + hipGetErrorString(e));
+    *ms_out = (double)ms / reps;
+    return PIMDB_OK;
+}
+
+extern "C" {
+
 int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_exchange, double* ms_step) {
     if (!ctx || reps < 1) return PIMDB_E_ARG;
-    float ms = 0.0f;
-    CK(hipStreamSynchronize(ctx->s0));
+    int rc = PIMDB_OK;
     if (ms_forces) {
-        CK(hipEventRecord(ctx->t0, ctx->s0));
-        for (int r = 0; r < reps; ++r) launch_bead_forces(ctx, ctx->s0);
-        CK(hipEventRecord(ctx->t1, ctx->s0));
-        CK(hipEventSynchronize(ctx->t1));
-        CK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-        *ms_forces = (double)ms / reps;
+        rc = time_captured(ctx, reps, [&]() { launch_bead_forces(ctx, ctx->s0); return PIMDB_OK; }, ms_forces);
+        if (rc) return rc;
     }
     if (ms_exchange) {
         *ms_exchange = 0.0;
         if (ctx->dv.bosonic) {
-            CK(hipEventRecord(ctx->t0, ctx->s0));
-            for (int r = 0; r < reps; ++r) {
+            rc = time_captured(ctx, reps, [&]() {
                 launch_exchange_forward(ctx, ctx->s0);
                 launch_exchange_forces(ctx, ctx->s0, nullptr, ctx->fex);
-            }
-            CK(hipEventRecord(ctx->t1, ctx->s0));
-            CK(hipEventSynchronize(ctx->t1));
-            CK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-            *ms_exchange = (double)ms / reps;
+                return PIMDB_OK;
+            }, ms_exchange);
+            if (rc) return rc;
         }
     }
     if (ms_step) {
-        CK(hipEventRecord(ctx->t0, ctx->s0));
-        for (int r = 0; r < reps; ++r) {
-            if (ctx->gexec) CK(hipGraphLaunch(ctx->gexec, ctx->s0));
-            else {
-                int rc = enqueue_step(ctx, nullptr, nullptr);
-                if (rc) return rc;
-            }
-        }
-        CK(hipEventRecord(ctx->t1, ctx->s0));
-        CK(hipEventSynchronize(ctx->t1));
-        CK(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-        *ms_step = (double)ms / reps;
-        int rc = finish_steps(ctx);
+        // whole steps (device noise), as the benchmark replays them; this
+        // advances the state by 2 * reps steps
+        rc = time_captured(ctx, reps, [&]() { return enqueue_step(ctx, nullptr, nullptr); }, ms_step);
+        if (rc) return rc;
+        rc = finish_steps(ctx);
         if (rc) return rc;
     }
     CK(hipGetLastError());

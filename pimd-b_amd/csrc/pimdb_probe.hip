@@ -1,0 +1,104 @@
+// pimdb_probe.hip -- micro-benchmarks for the roofline: FP64 FMA rate, FP64
+// exp throughput of the device math library, and HBM copy bandwidth.  The
+// copy kernel also serves as the known-size reference kernel when hardware
+// byte counters are scaled.
+#include <hip/hip_runtime.h>
+
+#include "pimdb_gpu.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_probe_fma(double* out, int iters, double a, double b) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = threadIdx.x * 1e-3 + k;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = fma(v[k], a, b);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+    if (s == 12345.678) out[0] = s;   // keeps the chains alive
+}
+
+__global__ void __launch_bounds__(256) k_probe_exp(double* out, int iters) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = -1e-3 * (threadIdx.x + k);
+    double s = 0.0;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double e = exp(v[k]);
+            s += e;
+            v[k] -= 1e-9;
+        }
+    }
+    if (s == 12345.678) out[0] = s;
+}
+
+__global__ void __launch_bounds__(256) k_probe_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gbs, long long copy_bytes) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return PIMDB_E_HIP;
+    const int cus = prop.multiProcessorCount;
+    hipEvent_t t0, t1;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return PIMDB_E_HIP;
+    double* sink = nullptr;
+    if (hipMalloc((void**)&sink, sizeof(double)) != hipSuccess) return PIMDB_E_HIP;
+    int rc = PIMDB_OK;
+    float ms = 0.0f;
+    const int blocks = cus * 16;
+    if (fma_tflops) {
+        const int iters = 4096;
+        hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(256), 0, 0, sink, 16, 1.0000001, 1e-9);
+        (void)hipEventRecord(t0, 0);
+        hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(256), 0, 0, sink, iters, 1.0000001, 1e-9);
+        (void)hipEventRecord(t1, 0);
+        if (hipEventSynchronize(t1) != hipSuccess) rc = PIMDB_E_HIP;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        *fma_tflops = 2.0 * 8.0 * iters * (double)blocks * 256 / (ms * 1e-3) / 1e12;
+    }
+    if (exp_gexp_per_s && !rc) {
+        const int iters = 256;
+        hipLaunchKernelGGL(k_probe_exp, dim3(blocks), dim3(256), 0, 0, sink, 4);
+        (void)hipEventRecord(t0, 0);
+        hipLaunchKernelGGL(k_probe_exp, dim3(blocks), dim3(256), 0, 0, sink, iters);
+        (void)hipEventRecord(t1, 0);
+        if (hipEventSynchronize(t1) != hipSuccess) rc = PIMDB_E_HIP;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        *exp_gexp_per_s = 4.0 * iters * (double)blocks * 256 / (ms * 1e-3) / 1e9;
+    }
+    if (copy_gbs && !rc) {
+        const size_t bytes = copy_bytes > 0 ? (size_t)copy_bytes : ((size_t)1 << 30);
+        double2 *a = nullptr, *b = nullptr;
+        if (hipMalloc((void**)&a, bytes) != hipSuccess || hipMalloc((void**)&b, bytes) != hipSuccess) {
+            rc = PIMDB_E_HIP;
+        } else {
+            const size_t n = bytes / sizeof(double2);
+            (void)hipMemset(a, 0, bytes);
+            hipLaunchKernelGGL(k_probe_copy, dim3(cus * 8), dim3(256), 0, 0, a, b, n);
+            (void)hipEventRecord(t0, 0);
+            const int reps = 5;
+            for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_probe_copy, dim3(cus * 8), dim3(256), 0, 0, a, b, n);
+            (void)hipEventRecord(t1, 0);
+            if (hipEventSynchronize(t1) != hipSuccess) rc = PIMDB_E_HIP;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            *copy_gbs = 2.0 * bytes * reps / (ms * 1e-3) / 1e9;
+        }
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    }
+    (void)hipFree(sink);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
+}

@@ -86,6 +86,7 @@ def lib():
         "pimdb_observables": (i, [vp, _DP]),
         "pimdb_graph_capture": (i, [vp]),
         "pimdb_time_kernels": (i, [vp, i, _DP, _DP, _DP]),
+        "pimdb_peak_probe": (i, [_DP, _DP, _DP, C.c_longlong]),
         "pimdb_xplan_create": (i, [i, i, d, d, i, d, C.POINTER(vp)]),
         "pimdb_xplan_destroy": (None, [vp]),
         "pimdb_xplan_forward": (i, [vp, vp, vp, vp, vp, vp, vp]),
@@ -188,6 +189,15 @@ class Context:
         vals = [np.zeros(1) if w else None for w in (forces, exchange, step)]
         self._chk(lib().pimdb_time_kernels(self.h, int(reps), *(_p(v) for v in vals)))
         return tuple(None if v is None else float(v[0]) for v in vals)
+
+
+def peak_probe(copy_bytes=1 << 30):
+    """Measured FP64 FMA TFLOP/s, FP64 exp Gexp/s and copy GB/s."""
+    v = [np.zeros(1) for _ in range(3)]
+    rc = lib().pimdb_peak_probe(_p(v[0]), _p(v[1]), _p(v[2]), int(copy_bytes))
+    if rc != OK:
+        _raise(rc, "pimdb_peak_probe failed")
+    return tuple(float(a[0]) for a in v)
 
 
 class XPlan:
