@@ -14,7 +14,9 @@ Forms measured:
            kernel per MD step (when the library provides it)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R]
-                       [--no-cpu-baseline] [--cpu-steps S]
+                       [--plain-steps S] [--no-cpu-baseline] [--cpu-steps S]
+The timed region is K batched steps (every replica advances K steps);
+`value` is replica-steps/s, the plain form is reported beside it.
 """
 import argparse
 import json
@@ -113,9 +115,34 @@ def bench_plain(kw, x, p, steps, warmup):
     return (t1 - t0) / steps * 1e3, ms_forces, ms_exch, ms_step
 
 
+def bench_batched(kw, x, p, replicas, steps, warmup):
+    """R copies of the workload (distinct noise seeds) in the persistent
+    kernel; returns wall ms per batched step, device ms per launch, tuning."""
+    params = []
+    for r in range(replicas):
+        k = dict(kw)
+        k["seed"] = kw["seed"] + 7919 * r
+        params.append(G.make_params(**k))
+    b = G.Batch(params)
+    for r in range(replicas):
+        b.set_state(r, x, p)
+    b.forces()
+    threads, resident = b.autotune(trial_steps=2)
+    b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)            # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    ms_launch = b.time_step(reps=5)
+    xs, _, fs = b.state(0)
+    ok = bool(np.all(np.isfinite(xs)) and np.all(np.isfinite(fs)))
+    b.close()
+    return (t1 - t0) / steps * 1e3, ms_launch, threads, resident, ok
+
+
 def cpu_baseline(kw, x, p, cpu_steps):
     """The oracle (single-thread C restatement of the reference path) on the
-    host cores of this machine, on the same initial state."""
+    host cores of this machine, on the same initial state; also the force
+    parity of both GPU forms on that state."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     sim = O.Sim(O.make_params(**kw))
@@ -125,25 +152,33 @@ def cpu_baseline(kw, x, p, cpu_steps):
     for _ in range(cpu_steps):
         sim.step()
     dt = time.perf_counter() - t0
-    # force parity on the benchmark's own initial state
     f_ref = O.update_forces(O.make_params(**kw), x)
+
+    def err(f):
+        return max(float(np.max(np.abs(f[b] - f_ref[b])) / np.max(np.abs(f_ref[b]))) for b in range(x.shape[0]))
+
     ctx = G.Context(G.make_params(**kw))
     ctx.set_state(x, p)
     ctx.forces()
-    _, _, f_gpu = ctx.state()
+    e_plain = err(ctx.state()[2])
     ctx.close()
-    err = max(float(np.max(np.abs(f_gpu[b] - f_ref[b])) / np.max(np.abs(f_ref[b]))) for b in range(x.shape[0]))
+    bt = G.Batch([G.make_params(**kw)])
+    bt.set_state(0, x, p)
+    bt.forces()
+    e_batch = err(bt.state(0)[2])
+    bt.close()
     return dict(value=cpu_steps / dt, unit="MD steps/s", cores=1, kind="port",
                 sample="%d OBABO steps of the benchmark workload from its initial state "
-                       "(oracle/pimdb_oracle.c, one thread)" % cpu_steps), err
+                       "(oracle/pimdb_oracle.c, one thread)" % cpu_steps), e_plain, e_batch
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--replicas", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=200, help="timed batched MD steps")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -157,37 +192,46 @@ def main():
     fm = flops_model(kw, n_pairs, n_in)
     fma_tf, exp_g, copy_gbs = G.peak_probe()
 
-    ms_step, ms_forces, ms_exch, ms_step_ev = bench_plain(kw, x, p, args.steps, args.warmup)
-    value = 1e3 / ms_step
+    ms_plain, ms_forces, ms_exch, ms_step_graph = bench_plain(kw, x, p, args.plain_steps, 50)
+    R = args.replicas
+    ms_batch, ms_launch, threads, resident, finite = bench_batched(kw, x, p, R, args.steps, args.warmup)
+    value = R * 1e3 / ms_batch
 
-    achieved = fm["force_kernel_flops"] / (ms_forces * 1e-3) / 1e12
-    t_roof_forces = fm["force_kernel_flops"] / (PEAK_FP64_TFLOPS * 1e12) + fm["force_kernel_exps"] / (exp_g * 1e9)
+    # dominant kernel: the persistent step kernel (one launch = one MD step of all R replicas)
+    fl = R * fm["step_flops"]
+    xp = R * fm["step_exps"]
+    achieved = fl / (ms_launch * 1e-3) / 1e12
+    t_roof = fl / (PEAK_FP64_TFLOPS * 1e12) + xp / (exp_g * 1e9)
     roofline = dict(bound="fp64", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                    kernel="k_bead_forces<3,AZIZ>", ms_per_launch=ms_forces,
-                    flops_per_launch=fm["force_kernel_flops"], exps_per_launch=fm["force_kernel_exps"],
-                    frac_with_exps=t_roof_forces / (ms_forces * 1e-3))
+                    kernel="k_batch_step<3,AZIZ> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
+                    flops_per_launch=fl, exps_per_launch=xp, frac_with_exps=t_roof / (ms_launch * 1e-3))
+    achieved_f = fm["force_kernel_flops"] / (ms_forces * 1e-3) / 1e12
 
     out = dict(
         metric="MD steps/s N=256 P=64 Aziz; batched/persistent vs plain GPU @force err<=1e-10",
         value=value, unit="MD steps/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=ms_step, higher_is_better=True, scaling="weak", vs_baseline=None,
+        ms_per_step=ms_batch, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic",
         config=dict(workload="N=256 He-4 bosons, P=64, D=3, T=1.5 K, Aziz, PBC 0.02186/A^3 "
                              "(L=42.914 bohr), cutoff L/2, dt=0.5 fs, NM Langevin, fixcom, FP64",
-                    form="plain (per-kernel, hipGraph replay)", natoms=256, nbeads=64,
-                    pairs_per_step=n_pairs, pairs_in_cutoff=n_in),
+                    form="batched persistent kernel + hipGraph, %d replicas (value = replica-steps/s)" % R,
+                    replicas=R, natoms=256, nbeads=64, pairs_per_step=n_pairs, pairs_in_cutoff=n_in,
+                    batch_threads=threads, batch_workgroups=resident),
         roofline=roofline,
-        stages_ms=dict(forces_kernel=ms_forces, exchange_chain=ms_exch, step_eager_events=ms_step_ev),
+        per_replica_steps_per_s=1e3 / ms_batch,
+        plain=dict(steps_per_s=1e3 / ms_plain, ms_per_step=ms_plain, steps=args.plain_steps,
+                   form="one system, per-kernel launches replayed from a hipGraph",
+                   forces_kernel_ms=ms_forces, forces_kernel_tflops=achieved_f,
+                   forces_kernel_frac=achieved_f / PEAK_FP64_TFLOPS, exchange_chain_ms=ms_exch,
+                   step_graph_ms=ms_step_graph),
         measured_peaks=dict(fp64_fma_tflops=fma_tf, fp64_exp_gexp_s=exp_g, copy_gbs=copy_gbs),
-        step_roofline=dict(flops=fm["step_flops"], exps=fm["step_exps"],
-                           frac=(fm["step_flops"] / (PEAK_FP64_TFLOPS * 1e12) + fm["step_exps"] / (exp_g * 1e9))
-                           / (ms_step * 1e-3)),
+        finite_state=finite,
     )
     if not args.no_cpu_baseline:
-        cb, err = cpu_baseline(kw, x, p, args.cpu_steps)
+        cb, e_plain, e_batch = cpu_baseline(kw, x, p, args.cpu_steps)
         out["cpu_baseline"] = cb
-        out["force_rel_err_vs_oracle"] = err
+        out["force_rel_err_vs_oracle"] = dict(plain=e_plain, batched=e_batch)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

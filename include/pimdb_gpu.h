@@ -139,6 +139,37 @@ int pimdb_xplan_forward(pimdb_xplan* plan, const double* x1, const double* xP, d
                         double* Vb, void* stream);
 int pimdb_xplan_vb_grad(pimdb_xplan* plan, const double* x1, const double* xP, double* grad, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Batched replicas: many independent systems of uneven N, P and T advanced */
+/* by one persistent kernel per MD step (device noise), replayed from a     */
+/* hipGraph.  All replicas share ndim and the interaction kind and use the  */
+/* cartesian propagator; each replica equals one independent reference run  */
+/* (the reference has no batched form).  A replica draws the device noise  */
+/* stream of its seed, so it follows pimdb_step of a single-system context  */
+/* with the same seed and state up to summation order.                      */
+typedef struct pimdb_batch pimdb_batch;
+int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out);
+void pimdb_batch_destroy(pimdb_batch* batch);
+/* Last failure of batch; with batch == NULL, the last failed create call. */
+const char* pimdb_batch_last_error(const pimdb_batch* batch);
+/* Per-replica state, host layout [bead][particle][axis]. */
+int pimdb_batch_set_state(pimdb_batch* batch, int rep, const double* x, const double* p);
+int pimdb_batch_get_state(pimdb_batch* batch, int rep, double* x, double* p, double* f);
+/* Forces of every replica (pimdb_forces for each). */
+int pimdb_batch_forces(pimdb_batch* batch);
+/* nsteps MD steps of every replica (pimdb_step with device noise for each). */
+int pimdb_batch_step(pimdb_batch* batch, int nsteps);
+/* Workgroup size of the persistent kernel (256, 512 or 1024; 0 keeps the
+ * current one) and number of workgroups (0 keeps the current one; default
+ * one per compute unit). */
+int pimdb_batch_configure(pimdb_batch* batch, int threads, int resident);
+/* Times trial_steps steps for each workgroup size and 1 or 2 workgroups per
+ * compute unit, keeps the fastest and restores the state it started from. */
+int pimdb_batch_autotune(pimdb_batch* batch, int trial_steps, int* threads_out, int* resident_out);
+/* Average device time of one batched MD step over reps graph replays
+ * (advances the state by reps + 1 steps). */
+int pimdb_batch_time_step(pimdb_batch* batch, int reps, double* ms_step);
+
 #ifdef __cplusplus
 }
 #endif

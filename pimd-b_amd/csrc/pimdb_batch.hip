@@ -1,0 +1,401 @@
+// pimdb_batch.hip -- C ABI of the batched persistent form (include/pimdb_gpu.h):
+// replicas of uneven N, P and T in one persistent kernel per MD step,
+// replayed from a hipGraph, with the workgroup size and residency chosen by a
+// small autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "pimdb_gpu.h"
+#include "batch.hpp"
+#include "ctx_common.hpp"
+
+using namespace pimdb;
+
+struct pimdb_batch {
+    std::vector<pimdb_params> prm;
+    std::vector<RepDesc> desc;
+    int R = 0, D = 3, kind = 0;
+    int threads = 1024, resident = 0;   // resident workgroups (0: all CUs)
+    int cus = 0;
+    size_t nx = 0, nE = 0, nV = 0, nC = 0;
+    size_t lds = 0;
+    BatchPtrs bp{};
+    RepDesc* d_desc = nullptr;
+    int *d_sched = nullptr, *d_sched_off = nullptr;
+    int grid = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+};
+
+#define BCK(expr)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return b->fail(PIMDB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Cost model for the schedule: ordered pairs dominate, the exchange adds N^2.
+static double rep_cost(const pimdb_params& p) {
+    const double N = p.natoms, P = p.nbeads;
+    return P * N * N + (p.bosonic && p.nbeads > 1 ? 8.0 * N * N : 0.0) + 4.0 * P * P * N;
+}
+
+// Longest-processing-time-first assignment of replicas to `grid` workgroups.
+static void build_schedule(pimdb_batch* b, std::vector<int>& sched, std::vector<int>& off) {
+    std::vector<int> order(b->R);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return rep_cost(b->prm[a]) > rep_cost(b->prm[c]); });
+    using Item = std::pair<double, int>;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+    for (int g = 0; g < b->grid; ++g) heap.push({0.0, g});
+    std::vector<std::vector<int>> lists(b->grid);
+    for (int r : order) {
+        Item it = heap.top();
+        heap.pop();
+        lists[it.second].push_back(r);
+        heap.push({it.first + rep_cost(b->prm[r]), it.second});
+    }
+    sched.clear();
+    off.assign(1, 0);
+    for (auto& l : lists) {
+        sched.insert(sched.end(), l.begin(), l.end());
+        off.push_back((int)sched.size());
+    }
+}
+
+static int upload_schedule(pimdb_batch* b) {
+    b->grid = std::max(1, std::min(b->R, b->resident > 0 ? b->resident : b->cus));
+    std::vector<int> sched, off;
+    build_schedule(b, sched, off);
+    if (b->d_sched) (void)hipFree(b->d_sched);
+    if (b->d_sched_off) (void)hipFree(b->d_sched_off);
+    b->d_sched = nullptr;
+    b->d_sched_off = nullptr;
+    BCK(hipMalloc((void**)&b->d_sched, sizeof(int) * sched.size()));
+    BCK(hipMalloc((void**)&b->d_sched_off, sizeof(int) * off.size()));
+    BCK(hipMemcpy(b->d_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->d_sched_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    b->bp.sched = b->d_sched;
+    b->bp.sched_off = b->d_sched_off;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    return PIMDB_OK;
+}
+
+template <int D>
+static void launch_batch_d(pimdb_batch* b, int mode) {
+    dim3 grid(b->grid), block(b->threads);
+    switch (b->kind) {
+    case POT_HARMONIC: hipLaunchKernelGGL((k_batch_step<D, POT_HARMONIC>), grid, block, b->lds, b->s, b->bp, mode); break;
+    case POT_DIPOLE: hipLaunchKernelGGL((k_batch_step<D, POT_DIPOLE>), grid, block, b->lds, b->s, b->bp, mode); break;
+    case POT_AZIZ: hipLaunchKernelGGL((k_batch_step<D, POT_AZIZ>), grid, block, b->lds, b->s, b->bp, mode); break;
+    default: hipLaunchKernelGGL((k_batch_step<D, POT_FREE>), grid, block, b->lds, b->s, b->bp, mode); break;
+    }
+}
+
+static int launch_batch(pimdb_batch* b, int mode) {
+    switch (b->D) {
+    case 1: launch_batch_d<1>(b, mode); break;
+    case 2: launch_batch_d<2>(b, mode); break;
+    default: launch_batch_d<3>(b, mode); break;
+    }
+    BCK(hipGetLastError());
+    return PIMDB_OK;
+}
+
+static int capture(pimdb_batch* b) {
+    if (b->gexec) return PIMDB_OK;
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+    int rc = launch_batch(b, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return b->fail(PIMDB_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    BCK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    return PIMDB_OK;
+}
+
+static int check_batch_status(pimdb_batch* b) {
+    std::vector<int> st(b->R);
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipMemcpy(st.data(), b->bp.status, sizeof(int) * b->R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < b->R; ++r) {
+        if (st[r]) {
+            BCK(hipMemset(b->bp.status, 0, sizeof(int) * b->R));
+            return b->fail(PIMDB_E_NONFINITE, "non-finite bosonic exchange potential in replica " + std::to_string(r));
+        }
+    }
+    return PIMDB_OK;
+}
+
+extern "C" {
+
+int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
+    if (!reps || !out || nrep < 1) return PIMDB_E_ARG;
+    *out = nullptr;
+    std::string why;
+    for (int r = 0; r < nrep; ++r) {
+        if (!validate_params(reps[r], &why)) {
+            set_create_error("replica " + std::to_string(r) + ": " + why);
+            return PIMDB_E_ARG;
+        }
+        if (reps[r].propagator != PIMDB_PROP_CARTESIAN) {
+            set_create_error("batched replicas use the cartesian propagator");
+            return PIMDB_E_ARG;
+        }
+        if (reps[r].ndim != reps[0].ndim || reps[r].int_kind != reps[0].int_kind) {
+            set_create_error("all replicas of a batch share ndim and the interaction kind");
+            return PIMDB_E_ARG;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_create_error("no HIP device available");
+        return PIMDB_E_HIP;
+    }
+    pimdb_batch* b = new pimdb_batch();
+    b->R = nrep;
+    b->D = reps[0].ndim;
+    b->kind = reps[0].int_kind;
+    b->prm.assign(reps, reps + nrep);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        delete b;
+        set_create_error("hipGetDeviceProperties failed");
+        return PIMDB_E_HIP;
+    }
+    b->cus = prop.multiProcessorCount;
+    int Nmax = 1;
+    for (int r = 0; r < nrep; ++r) {
+        const pimdb_params& p = reps[r];
+        const Derived dv = derive(p);
+        RepDesc d{};
+        d.N = p.natoms; d.P = p.nbeads; d.bosonic = dv.bosonic; d.pbc = p.pbc;
+        d.thermostat = p.thermostat; d.nm = p.nmthermostat; d.fixcom = p.fixcom;
+        d.ext_harmonic = p.ext_kind == PIMDB_POT_HARMONIC;
+        d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
+        d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
+        d.ext_k = dv.ext_k; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
+        d.seed = p.seed; d.rid = 0;   // noise stream keyed by the seed, as in the single-system form
+        d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
+        d.off_E = b->nE; b->nE += (size_t)p.natoms * (p.natoms + 1) / 2;
+        d.off_V = b->nV; b->nV += (size_t)p.natoms + 1;
+        d.off_C = b->nC; b->nC += (size_t)p.nbeads * p.nbeads;
+        b->desc.push_back(d);
+        Nmax = std::max(Nmax, p.natoms);
+    }
+    b->lds = sizeof(double) * (16 + rec_shared_doubles(Nmax));
+    int rc = PIMDB_OK;
+    if (b->lds > device_lds_limit()) rc = b->fail(PIMDB_E_ARG, "natoms too large for the batched exchange");
+    auto alloc = [&](double** ptr, size_t n) {
+        if (rc) return;
+        if (hipMalloc((void**)ptr, std::max<size_t>(1, n) * sizeof(double)) != hipSuccess ||
+            hipMemset(*ptr, 0, std::max<size_t>(1, n) * sizeof(double)) != hipSuccess)
+            rc = b->fail(PIMDB_E_HIP, "device allocation failed");
+    };
+    double *Ct = nullptr, *CTt = nullptr;
+    alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx); alloc(&b->bp.pnm, b->nx);
+    alloc(&b->bp.E, b->nE); alloc(&b->bp.ET, b->nE); alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
+    alloc(&Ct, b->nC); alloc(&CTt, b->nC);
+    b->bp.Ct = Ct;
+    b->bp.CTt = CTt;
+    if (!rc && (hipMalloc((void**)&b->bp.status, sizeof(int) * nrep) != hipSuccess ||
+                hipMemset(b->bp.status, 0, sizeof(int) * nrep) != hipSuccess ||
+                hipMalloc((void**)&b->bp.ctl, 2 * sizeof(unsigned long long)) != hipSuccess ||
+                hipMemset(b->bp.ctl, 0, 2 * sizeof(unsigned long long)) != hipSuccess ||
+                hipMalloc((void**)&b->d_desc, sizeof(RepDesc) * nrep) != hipSuccess ||
+                hipMemcpy(b->d_desc, b->desc.data(), sizeof(RepDesc) * nrep, hipMemcpyHostToDevice) != hipSuccess))
+        rc = b->fail(PIMDB_E_HIP, "device allocation failed");
+    if (!rc) {
+        b->bp.reps = b->d_desc;
+        std::vector<double> hc(b->nC), hct(b->nC);
+        for (int r = 0; r < nrep && !rc; ++r) {
+            const int P = reps[r].nbeads;
+            std::vector<double> C0((size_t)P * P), CT0((size_t)P * P);
+            nm_matrices(P, C0.data(), CT0.data());
+            transpose_square(P, C0.data(), hc.data() + b->desc[r].off_C);
+            transpose_square(P, CT0.data(), hct.data() + b->desc[r].off_C);
+        }
+        if (hipMemcpy(Ct, hc.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(CTt, hct.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess)
+            rc = b->fail(PIMDB_E_HIP, "upload of normal-mode tables failed");
+    }
+    if (!rc && (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess))
+        rc = b->fail(PIMDB_E_HIP, "stream/event creation failed");
+    if (!rc) rc = upload_schedule(b);
+    if (rc) {
+        set_create_error(b->err);
+        pimdb_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PIMDB_OK;
+}
+
+void pimdb_batch_destroy(pimdb_batch* b) {
+    if (!b) return;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.pnm, b->bp.E, b->bp.ET, b->bp.V, b->bp.Vb,
+                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
+    for (double* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->bp.status) (void)hipFree(b->bp.status);
+    if (b->bp.ctl) (void)hipFree(b->bp.ctl);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_sched) (void)hipFree(b->d_sched);
+    if (b->d_sched_off) (void)hipFree(b->d_sched_off);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+const char* pimdb_batch_last_error(const pimdb_batch* b) { return b ? b->err.c_str() : create_error(); }
+
+int pimdb_batch_set_state(pimdb_batch* b, int rep, const double* x, const double* p) {
+    if (!b || rep < 0 || rep >= b->R) return PIMDB_E_ARG;
+    const pimdb_params& q = b->prm[rep];
+    const size_t n = (size_t)q.nbeads * q.ndim * q.natoms;
+    std::vector<double> st(n);
+    const double* src[2] = {x, p};
+    double* dst[2] = {b->bp.x, b->bp.p};
+    for (int k = 0; k < 2; ++k) {
+        if (!src[k]) continue;
+        to_soa(src[k], st.data(), q.natoms, q.nbeads, q.ndim);
+        BCK(hipMemcpyAsync(dst[k] + b->desc[rep].off_x, st.data(), sizeof(double) * n, hipMemcpyHostToDevice, b->s));
+        BCK(hipStreamSynchronize(b->s));
+    }
+    return PIMDB_OK;
+}
+
+int pimdb_batch_get_state(pimdb_batch* b, int rep, double* x, double* p, double* f) {
+    if (!b || rep < 0 || rep >= b->R) return PIMDB_E_ARG;
+    const pimdb_params& q = b->prm[rep];
+    const size_t n = (size_t)q.nbeads * q.ndim * q.natoms;
+    std::vector<double> st(n);
+    double* dst[3] = {x, p, f};
+    const double* src[3] = {b->bp.x, b->bp.p, b->bp.f};
+    for (int k = 0; k < 3; ++k) {
+        if (!dst[k]) continue;
+        BCK(hipMemcpyAsync(st.data(), src[k] + b->desc[rep].off_x, sizeof(double) * n, hipMemcpyDeviceToHost, b->s));
+        BCK(hipStreamSynchronize(b->s));
+        from_soa(st.data(), dst[k], q.natoms, q.nbeads, q.ndim);
+    }
+    return PIMDB_OK;
+}
+
+int pimdb_batch_forces(pimdb_batch* b) {
+    if (!b) return PIMDB_E_ARG;
+    int rc = launch_batch(b, 1);
+    if (rc) return rc;
+    return check_batch_status(b);
+}
+
+int pimdb_batch_step(pimdb_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return PIMDB_E_ARG;
+    int rc = capture(b);
+    if (rc) return rc;
+    for (int s = 0; s < nsteps; ++s) BCK(hipGraphLaunch(b->gexec, b->s));
+    return check_batch_status(b);
+}
+
+int pimdb_batch_configure(pimdb_batch* b, int threads, int resident) {
+    if (!b) return PIMDB_E_ARG;
+    if (threads) {
+        if (threads != 256 && threads != 512 && threads != 1024) return b->fail(PIMDB_E_ARG, "threads must be 256, 512 or 1024");
+        b->threads = threads;
+    }
+    if (resident < 0) return b->fail(PIMDB_E_ARG, "resident must be >= 0");
+    if (resident) b->resident = resident;
+    BCK(hipStreamSynchronize(b->s));
+    return upload_schedule(b);
+}
+
+int pimdb_batch_time_step(pimdb_batch* b, int reps, double* ms_step) {
+    if (!b || reps < 1 || !ms_step) return PIMDB_E_ARG;
+    int rc = capture(b);
+    if (rc) return rc;
+    BCK(hipGraphLaunch(b->gexec, b->s));
+    BCK(hipEventRecord(b->t0, b->s));
+    for (int r = 0; r < reps; ++r) BCK(hipGraphLaunch(b->gexec, b->s));
+    BCK(hipEventRecord(b->t1, b->s));
+    BCK(hipEventSynchronize(b->t1));
+    float ms = 0.0f;
+    BCK(hipEventElapsedTime(&ms, b->t0, b->t1));
+    *ms_step = (double)ms / reps;
+    return check_batch_status(b);
+}
+
+// Tries every workgroup size and residency (1 or 2 workgroups per CU) for
+// trial_steps steps from a saved state, keeps the fastest and restores the
+// state, so the batch's trajectory is unaffected.
+int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int* resident_out) {
+    if (!b || trial_steps < 1) return PIMDB_E_ARG;
+    double *sx = nullptr, *sp = nullptr, *sf = nullptr;
+    unsigned long long sctl[2];
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipMalloc((void**)&sx, sizeof(double) * b->nx));
+    BCK(hipMalloc((void**)&sp, sizeof(double) * b->nx));
+    BCK(hipMalloc((void**)&sf, sizeof(double) * b->nx));
+    BCK(hipMemcpy(sx, b->bp.x, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
+    const int cand_threads[3] = {256, 512, 1024};
+    double best = 1e300;
+    int bt = b->threads, br = b->resident;
+    int rc = PIMDB_OK;
+    for (int ti = 0; ti < 3 && !rc; ++ti) {
+        for (int per_cu = 1; per_cu <= 2 && !rc; ++per_cu) {
+            b->threads = cand_threads[ti];
+            b->resident = b->cus * per_cu;
+            rc = upload_schedule(b);
+            double ms = 0.0;
+            if (!rc) rc = pimdb_batch_time_step(b, trial_steps, &ms);
+            if (!rc && ms < best) {
+                best = ms;
+                bt = b->threads;
+                br = b->resident;
+            }
+        }
+    }
+    b->threads = bt;
+    b->resident = br;
+    if (!rc) rc = upload_schedule(b);
+    BCK(hipMemcpy(b->bp.x, sx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
+    (void)hipFree(sx);
+    (void)hipFree(sp);
+    (void)hipFree(sf);
+    if (threads_out) *threads_out = b->threads;
+    if (resident_out) *resident_out = b->resident;
+    return rc;
+}
+
+}  // extern "C"

@@ -91,6 +91,16 @@ def lib():
         "pimdb_xplan_destroy": (None, [vp]),
         "pimdb_xplan_forward": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "pimdb_xplan_vb_grad": (i, [vp, vp, vp, vp, vp]),
+        "pimdb_batch_create": (i, [C.POINTER(Params), i, C.POINTER(vp)]),
+        "pimdb_batch_destroy": (None, [vp]),
+        "pimdb_batch_last_error": (C.c_char_p, [vp]),
+        "pimdb_batch_set_state": (i, [vp, i, _DP, _DP]),
+        "pimdb_batch_get_state": (i, [vp, i, _DP, _DP, _DP]),
+        "pimdb_batch_forces": (i, [vp]),
+        "pimdb_batch_step": (i, [vp, i]),
+        "pimdb_batch_configure": (i, [vp, i, i]),
+        "pimdb_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
+        "pimdb_batch_time_step": (i, [vp, i, _DP]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -189,6 +199,66 @@ class Context:
         vals = [np.zeros(1) if w else None for w in (forces, exchange, step)]
         self._chk(lib().pimdb_time_kernels(self.h, int(reps), *(_p(v) for v in vals)))
         return tuple(None if v is None else float(v[0]) for v in vals)
+
+
+class Batch:
+    """Independent replicas advanced by one persistent kernel per MD step."""
+
+    def __init__(self, params_list):
+        arr = (Params * len(params_list))(*params_list)
+        h = C.c_void_p()
+        rc = lib().pimdb_batch_create(arr, len(params_list), C.byref(h))
+        if rc != OK:
+            _raise(rc, lib().pimdb_batch_last_error(None).decode())
+        self.h = h
+        self.params = list(params_list)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pimdb_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, rc):
+        if rc != OK:
+            _raise(rc, lib().pimdb_batch_last_error(self.h).decode())
+
+    def shape(self, r):
+        q = self.params[r]
+        return (q.nbeads, q.natoms, q.ndim)
+
+    def set_state(self, r, x=None, p=None):
+        x = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        for a in (x, p):
+            if a is not None and a.shape != self.shape(r):
+                raise ValueError("state of replica %d must have shape %s" % (r, self.shape(r)))
+        self._chk(lib().pimdb_batch_set_state(self.h, int(r), _p(x), _p(p)))
+
+    def state(self, r):
+        x, p, f = (np.zeros(self.shape(r)) for _ in range(3))
+        self._chk(lib().pimdb_batch_get_state(self.h, int(r), _p(x), _p(p), _p(f)))
+        return x, p, f
+
+    def forces(self):
+        self._chk(lib().pimdb_batch_forces(self.h))
+
+    def step(self, nsteps=1):
+        self._chk(lib().pimdb_batch_step(self.h, int(nsteps)))
+
+    def configure(self, threads=0, resident=0):
+        self._chk(lib().pimdb_batch_configure(self.h, int(threads), int(resident)))
+
+    def autotune(self, trial_steps=3):
+        t, r = C.c_int(), C.c_int()
+        self._chk(lib().pimdb_batch_autotune(self.h, int(trial_steps), C.byref(t), C.byref(r)))
+        return t.value, r.value
+
+    def time_step(self, reps=5):
+        v = np.zeros(1)
+        self._chk(lib().pimdb_batch_time_step(self.h, int(reps), _p(v)))
+        return float(v[0])
 
 
 def peak_probe(copy_bytes=1 << 30):

@@ -1,0 +1,125 @@
+"""Batched persistent form: every replica's forces against the oracle, every
+replica's trajectory against a single-system context with the same seed and
+state (device noise), graph determinism and the autotuner's state restore."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pimdb_amd as G
+from helpers import AMU, KELVIN, MEV, aziz_config, per_bead_err, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def dipole_replicas(n, seed=0):
+    """Config-4-like replicas: N in {8..128}, P in {16,32,64}, T in [1,4] K,
+    D=2, trap 3 meV + dipole pairs, all pairs; particles on a jittered 4-bohr
+    lattice so the 1/r^4 dipole forces stay integrable at dt = 20 a.u."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(n):
+        N = int(rng.choice(np.arange(8, 129, 8)))
+        P = int(rng.choice([16, 32, 64]))
+        T = rng.uniform(1.0, 4.0)
+        kw = dict(natoms=N, nbeads=P, ndim=2, mass=4.0026 * AMU, temperature=T * KELVIN, dt=20.0,
+                  gamma=1.0 / 2000.0, bosonic=1, pbc=0, size=60.0, ext_kind="harmonic", ext_omega=3.0 * MEV,
+                  int_kind="dipole", int_strength=rng.uniform(0.5, 2.0), cutoff=-1.0, propagator="cartesian",
+                  thermostat="langevin", nmthermostat=int(r % 2), fixcom=int(r % 3 == 0), seed=1000 + r)
+        side = int(np.ceil(np.sqrt(N)))
+        grid = np.array([(a, c) for a in range(side) for c in range(side)][:N], dtype=float) * 4.0
+        centres = grid - grid.mean(axis=0) + rng.uniform(-0.5, 0.5, size=(N, 2))   # >= 3 bohr apart
+        x = np.repeat(centres[None], P, axis=0) + rng.normal(0, 0.3, size=(P, N, 2))
+        p = rng.normal(0, 0.5, size=(P, N, 2))
+        out.append((kw, x, p))
+    return out
+
+
+def aziz_replicas(n):
+    out = []
+    for r in range(n):
+        kw, x = aziz_config(int([32, 64, 48, 96][r % 4]), int([8, 16, 32, 8][r % 4]), 1.5 + 0.2 * r,
+                            replicate=False, seed=40 + r)
+        kw["seed"] = 77 + r
+        p = np.random.default_rng(r).normal(0, 0.3, size=x.shape)
+        out.append((kw, x, p))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz"])
+def test_batch_forces_match_oracle(which):
+    reps = dipole_replicas(12, seed=3) if which == "dipole" else aziz_replicas(6)
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    for r, (kw, x, p) in enumerate(reps):
+        _, _, f = b.state(r)
+        fo = O.update_forces(O.make_params(**kw), x)
+        assert per_bead_err(f, fo) < 1e-10, r
+    b.close()
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz"])
+def test_batch_follows_single_system(which):
+    reps = dipole_replicas(6, seed=5) if which == "dipole" else aziz_replicas(4)
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    ctxs = []
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+        c = G.Context(G.make_params(**kw))
+        c.set_state(x, p)
+        c.forces()
+        ctxs.append(c)
+    b.forces()
+    b.step(15)
+    for r, c in enumerate(ctxs):
+        c.step(15)
+        xb, pb, fb = b.state(r)
+        xc, pc, fc = c.state()
+        assert rel_err(xb, xc) < 1e-10, r
+        assert rel_err(pb, pc) < 1e-9, r
+        assert per_bead_err(fb, fc) < 1e-9, r
+        c.close()
+    b.close()
+
+
+def test_batch_graph_replays_are_deterministic():
+    reps = dipole_replicas(9, seed=8)
+    bs = [G.Batch([G.make_params(**kw) for kw, _, _ in reps]) for _ in range(2)]
+    for b in bs:
+        for r, (kw, x, p) in enumerate(reps):
+            b.set_state(r, x, p)
+        b.forces()
+    for rep in range(3):
+        for b in bs:
+            b.step(4)
+        for r in range(len(reps)):
+            sa, sb = bs[0].state(r), bs[1].state(r)
+            assert all(np.array_equal(u, v) for u, v in zip(sa, sb)), (rep, r)
+    for b in bs:
+        b.close()
+
+
+def test_batch_autotune_restores_state():
+    reps = dipole_replicas(5, seed=9)
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    before = [b.state(r) for r in range(len(reps))]
+    threads, resident = b.autotune(trial_steps=2)
+    assert threads in (256, 512, 1024) and resident >= 1
+    after = [b.state(r) for r in range(len(reps))]
+    for s0, s1 in zip(before, after):
+        assert all(np.array_equal(u, v) for u, v in zip(s0, s1))
+    b.step(3)
+    b.close()
+
+
+def test_batch_rejects_mixed_dimensions():
+    reps = dipole_replicas(2, seed=1)
+    kw = dict(reps[1][0])
+    kw["ndim"] = 3
+    with pytest.raises(G.PimdbError) as ei:
+        G.Batch([G.make_params(**reps[0][0]), G.make_params(**kw)])
+    assert ei.value.code == G.E_ARG
