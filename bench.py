@@ -133,10 +133,11 @@ def bench_batched(kw, x, p, replicas, steps, warmup):
     b.step(steps)            # returns after the stream is synchronised
     t1 = time.perf_counter()
     ms_launch = b.time_step(reps=5)
+    phases = dict(pairs=b.time_phase(1), exchange=b.time_phase(2), integrator=b.time_phase(3))
     xs, _, fs = b.state(0)
     ok = bool(np.all(np.isfinite(xs)) and np.all(np.isfinite(fs)))
     b.close()
-    return (t1 - t0) / steps * 1e3, ms_launch, threads, resident, ok
+    return (t1 - t0) / steps * 1e3, ms_launch, threads, resident, ok, phases
 
 
 def cpu_baseline(kw, x, p, cpu_steps):
@@ -194,7 +195,7 @@ def main():
 
     ms_plain, ms_forces, ms_exch, ms_step_graph = bench_plain(kw, x, p, args.plain_steps, 50)
     R = args.replicas
-    ms_batch, ms_launch, threads, resident, finite = bench_batched(kw, x, p, R, args.steps, args.warmup)
+    ms_batch, ms_launch, threads, resident, finite, phases = bench_batched(kw, x, p, R, args.steps, args.warmup)
     value = R * 1e3 / ms_batch
 
     # dominant kernel: the persistent step kernel (one launch = one MD step of all R replicas)
@@ -207,6 +208,15 @@ def main():
                     kernel="k_batch_step<3,AZIZ> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
                     flops_per_launch=fl, exps_per_launch=xp, frac_with_exps=t_roof / (ms_launch * 1e-3))
     achieved_f = fm["force_kernel_flops"] / (ms_forces * 1e-3) / 1e12
+
+    def frac(flops, exps, ms):
+        return (flops / (PEAK_FP64_TFLOPS * 1e12) + exps / (exp_g * 1e9)) / (ms * 1e-3)
+
+    batch_phases = dict(
+        ms=phases,
+        pairs_frac=frac(R * fm["force_kernel_flops"], R * fm["force_kernel_exps"], phases["pairs"]),
+        exchange_frac=frac(R * fm["exchange_flops"], R * fm["exchange_exps"], phases["exchange"]),
+        exchange_tflops=R * fm["exchange_flops"] / (phases["exchange"] * 1e-3) / 1e12)
 
     out = dict(
         metric="MD steps/s N=256 P=64 Aziz; batched/persistent vs plain GPU @force err<=1e-10",
@@ -225,6 +235,7 @@ def main():
                    forces_kernel_ms=ms_forces, forces_kernel_tflops=achieved_f,
                    forces_kernel_frac=achieved_f / PEAK_FP64_TFLOPS, exchange_chain_ms=ms_exch,
                    step_graph_ms=ms_step_graph),
+        batched_phases=batch_phases,
         measured_peaks=dict(fp64_fma_tflops=fma_tf, fp64_exp_gexp_s=exp_g, copy_gbs=copy_gbs),
         finite_state=finite,
     )

@@ -169,6 +169,11 @@ int pimdb_batch_autotune(pimdb_batch* batch, int trial_steps, int* threads_out, 
 /* Average device time of one batched MD step over reps graph replays
  * (advances the state by reps + 1 steps). */
 int pimdb_batch_time_step(pimdb_batch* batch, int reps, double* ms_step);
+/* Average device time of one launch doing only phase 0 (full step), 1 (pair,
+ * trap and spring forces), 2 (exchange: cycle energies, recursions, exterior
+ * forces) or 3 (O-steps, kicks and drift) for every replica; the state is
+ * saved before and restored after. */
+int pimdb_batch_time_phase(pimdb_batch* batch, int phase, int reps, double* ms);
 
 #ifdef __cplusplus
 }

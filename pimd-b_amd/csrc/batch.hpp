@@ -25,14 +25,14 @@ struct RepDesc {
     double k, bt, c1, c2;
     double ext_k, int_k, strength, cutoff;
     uint32_t seed, rid;
-    unsigned long long off_x;   // P*D*N block of x, p, f, pnm
+    unsigned long long off_x;   // P*D*N block of x, p, f
     unsigned long long off_E;   // N(N+1)/2 block of E, ET
     unsigned long long off_V;   // N+1 block of V, Vb
     unsigned long long off_C;   // P*P block of Ct, CTt
 };
 
 struct BatchPtrs {
-    double *x, *p, *f, *pnm, *E, *ET, *V, *Vb;
+    double *x, *p, *f, *E, *ET, *V, *Vb;
     const double *Ct, *CTt;
     const RepDesc* reps;
     const int* sched;          // replicas of workgroup g: sched[sched_off[g] .. sched_off[g+1])
@@ -58,12 +58,11 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
 // simulation.cpp:581-603).
 template <int D>
 __device__ __noinline__ void wg_momenta(const RepDesc& r, const BatchPtrs& bp, bool kick, uint32_t step, uint32_t ostep,
-                           double* red) {
+                                        double* red, double* lds) {
     const int N = r.N, P = r.P;
     const int tot = P * D * N;
     double* p = bp.p + r.off_x;
     const double* f = bp.f + r.off_x;
-    double* pnm = bp.pnm + r.off_x;
     const double half_dt = 0.5 * r.dt;
     if (r.thermostat != 1 || !r.nm) {
         for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -83,21 +82,36 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r, const BatchPtrs& bp, b
         }
         const double* Ct = bp.Ct + r.off_C;
         const double* CTt = bp.CTt + r.off_C;
-        // mode k of column (a, i): sum_j C[k][j] p[j][a][i], then the Langevin update
-        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-            const int k = e / (D * N), col = e % (D * N);
-            double s = 0.0;
-            for (int j = 0; j < P; ++j) s += Ct[(size_t)j * P + k] * p[(size_t)j * D * N + col];
-            double g0, g1;
-            gauss2((uint32_t)e, step, ostep, r.rid, r.seed, g0, g1);
-            pnm[e] = r.c1 * s + r.c2 * g0;
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-            const int b = e / (D * N), col = e % (D * N);
-            double s = 0.0;
-            for (int k = 0; k < P; ++k) s += CTt[(size_t)k * P + b] * pnm[(size_t)k * D * N + col];
-            p[e] = s;
+        // Columns (a, i) in tiles of 64 staged in LDS; wave w computes modes
+        // k = w, w + nwave, ... for the 64 columns of the tile (lane = column),
+        // so C[k][j] is the same for the whole wave.
+        const int ncols = D * N;
+        const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+        double* pc = lds;                        // [P][64]
+        double* pn = lds + (size_t)P * WAVE;     // [P][64]
+        for (int c0 = 0; c0 < ncols; c0 += WAVE) {
+            const int col = c0 + lane;
+            const bool valid = col < ncols;
+            for (int e = threadIdx.x; e < P * WAVE; e += blockDim.x) {
+                const int j = e / WAVE, c = c0 + e % WAVE;
+                pc[e] = c < ncols ? p[(size_t)j * ncols + c] : 0.0;
+            }
+            __syncthreads();
+            // mode k of column (a, i): sum_j C[k][j] p[j][a][i], then the Langevin update
+            for (int k = wave; k < P; k += nw) {
+                double s = 0.0;
+                for (int j = 0; j < P; ++j) s += Ct[(size_t)j * P + k] * pc[j * WAVE + lane];
+                double g0 = 0.0, g1;
+                if (valid) gauss2((uint32_t)((size_t)k * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                pn[k * WAVE + lane] = r.c1 * s + r.c2 * g0;
+            }
+            __syncthreads();
+            for (int b = wave; b < P; b += nw) {
+                double s = 0.0;
+                for (int k = 0; k < P; ++k) s += CTt[(size_t)k * P + b] * pn[k * WAVE + lane];
+                if (valid) p[(size_t)b * ncols + col] = s;
+            }
+            __syncthreads();
         }
     }
     if (r.fixcom) {
@@ -133,56 +147,98 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r, const BatchPtrs& bp
     __syncthreads();
 }
 
+// Doubles of LDS the force phase stages positions in (a group of beads).
+constexpr int BATCH_POS_LDS = 8192;
+
 // Physical forces and interior springs on every (bead, particle) of a replica
-// (simulation.cpp:353-455).  Work items (b, i) with i fastest: the 64 lanes
-// of a wave read the same partner position, a broadcast.
+// (simulation.cpp:353-455).  Beads are taken in groups whose positions fit in
+// LDS; work items (bead of the group, particle i) with i fastest, each running
+// a slice of the partners j from LDS; slices are summed in LDS.
 template <int D, int KIND>
-__device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& bp) {
+__device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& bp, double* lds) {
     const int N = r.N, P = r.P;
     const double* x = bp.x + r.off_x;
     double* f = bp.f + r.off_x;
     const bool pbc = r.pbc != 0;
     const size_t bs = (size_t)D * N;
-    for (int item = threadIdx.x; item < P * N; item += blockDim.x) {
-        const int b = item / N, i = item % N;
-        const double* xb = x + bs * b;
-        double xi[D], acc[D];
+    const int G = max(1, min(P, BATCH_POS_LDS / (D * N)));
+    double* sx = lds;                                // [G][D][N]
+    double* part = lds + (size_t)G * D * N;          // [nslice-1][D][G*N]
+    for (int b0 = 0; b0 < P; b0 += G) {
+        const int g = min(G, P - b0);
+        const int items = g * N;
+        const int nslice = max(1, min((int)blockDim.x / items, N));
+        for (int e = threadIdx.x; e < g * D * N; e += blockDim.x) sx[e] = x[bs * b0 + e];
+        __syncthreads();
+        const int chunk = (N + nslice - 1) / nslice;
+        for (int t = threadIdx.x; t < items * nslice; t += blockDim.x) {
+            const int item = t % items, slice = t / items;
+            const int bl = item / N, i = item % N;
+            const double* xb = sx + (size_t)bl * D * N;
+            double xi[D], acc[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) { xi[a] = xb[(size_t)a * N + i]; acc[a] = 0.0; }
-        if constexpr (KIND != POT_FREE) {
-            if (r.cutoff != 0.0) {
-                for (int j = 0; j < N; ++j) {
-                    if (j == i) continue;
-                    double d[D];
+            for (int a = 0; a < D; ++a) { xi[a] = xb[a * N + i]; acc[a] = 0.0; }
+            if constexpr (KIND != POT_FREE) {
+                if (r.cutoff != 0.0) {
+                    const int jlo = slice * chunk, jhi = min(N, jlo + chunk);
+                    for (int j = jlo; j < jhi; ++j) {
+                        if (j == i) continue;
+                        double d[D];
 #pragma unroll
-                    for (int a = 0; a < D; ++a) d[a] = disp_fast(xb[(size_t)a * N + j], xi[a], pbc, r.L, r.invL);
-                    const double r2 = norm2<D>(d);
-                    const double norm = sqrt(r2);
-                    if (norm < r.cutoff || r.cutoff < 0.0) {
-                        const double g = pair_grad_factor_fast<KIND>(norm, 1.0 / norm, r.int_k, r.strength);
+                        for (int a = 0; a < D; ++a) d[a] = disp_fast(xb[a * N + j], xi[a], pbc, r.L, r.invL);
+                        const double r2 = norm2<D>(d);
+                        const double norm = sqrt(r2);
+                        if (norm < r.cutoff || r.cutoff < 0.0) {
+                            const double gg = pair_grad_factor_fast<KIND>(norm, 1.0 / norm, r.int_k, r.strength);
 #pragma unroll
-                        for (int a = 0; a < D; ++a) acc[a] += -(g * d[a]);
+                            for (int a = 0; a < D; ++a) acc[a] += -(gg * d[a]);
+                        }
                     }
                 }
             }
-        }
-        const bool exchange_bead = r.bosonic && (b == 0 || b == P - 1);
-        const double* xprev = x + bs * ((b - 1 + P) % P);
-        const double* xnext = x + bs * ((b + 1) % P);
+            if (slice > 0) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const double ext = r.ext_harmonic ? (xi[a] * r.ext_k) * (-1.0) : -0.0;
-            const double phys = ext + acc[a];
-            double spring = 0.0;
-            if (!exchange_bead) {
-                const double dp = disp(xi[a], xprev[(size_t)a * N + i], pbc, r.L);
-                const double dn = disp(xi[a], xnext[(size_t)a * N + i], pbc, r.L);
-                spring = r.k * (dp + dn);
+                for (int a = 0; a < D; ++a) part[((size_t)(slice - 1) * D + a) * items + item] = acc[a];
+            } else {
+#pragma unroll
+                for (int a = 0; a < D; ++a) f[bs * b0 + (size_t)bl * bs + (size_t)a * N + i] = acc[a];
             }
-            f[bs * b + (size_t)a * N + i] = exchange_bead ? phys : spring + phys;
         }
+        __syncthreads();
+        // sum the slices, add the trap and the interior springs
+        for (int item = threadIdx.x; item < items; item += blockDim.x) {
+            const int bl = item / N, i = item % N, b = b0 + bl;
+            const bool exchange_bead = r.bosonic && (b == 0 || b == P - 1);
+            const double* xprev = x + bs * ((b - 1 + P) % P);
+            const double* xnext = x + bs * ((b + 1) % P);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const size_t idx = bs * b + (size_t)a * N + i;
+                double acc = f[idx];
+                for (int sl = 1; sl < nslice; ++sl) acc += part[((size_t)(sl - 1) * D + a) * items + item];
+                const double xi = sx[(size_t)bl * D * N + a * N + i];
+                const double ext = r.ext_harmonic ? (xi * r.ext_k) * (-1.0) : -0.0;
+                const double phys = ext + acc;
+                double spring = 0.0;
+                if (!exchange_bead) {
+                    const double dp = disp(xi, xprev[(size_t)a * N + i], pbc, r.L);
+                    const double dn = disp(xi, xnext[(size_t)a * N + i], pbc, r.L);
+                    spring = r.k * (dp + dn);
+                }
+                f[idx] = exchange_bead ? phys : spring + phys;
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+}
+
+// LDS doubles the force phase needs for a replica of N particles in D axes
+// (positions of a bead group plus the slice partials).
+__host__ __device__ inline size_t batch_force_lds_doubles(int N, int P, int D, int threads) {
+    const int G = BATCH_POS_LDS / (D * N) > 1 ? (BATCH_POS_LDS / (D * N) < P ? BATCH_POS_LDS / (D * N) : P) : 1;
+    const size_t forces = (size_t)G * D * N + (size_t)threads * D;
+    const size_t nm = 2 * (size_t)P * WAVE;   // normal-mode column tiles
+    return forces > nm ? forces : nm;
 }
 
 // Exchange of one replica: cycle energies, both recursions, exterior forces
@@ -224,26 +280,32 @@ __device__ __noinline__ void wg_exchange(const RepDesc& r, const BatchPtrs& bp, 
     __syncthreads();
 }
 
-// One MD step of every replica (mode 0), or forces only (mode 1).
-template <int D, int KIND>
-static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp, int mode) {
+// Work of one launch: a full MD step, or parts of it for timing.
+enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3, BM_INTEGRATOR = 4 };
+
+// One MD step of every replica (BM_STEP), the forces only (BM_FORCES), or one
+// phase for timing.  MAXT is the largest workgroup size the instantiation is
+// compiled for (it sets the register budget).
+template <int D, int KIND, int MAXT>
+static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mode) {
     extern __shared__ double smem[];
     double* red = smem;                 // [16]
-    double* lds = smem + 16;            // recursion workspace
+    double* lds = smem + 16;            // force staging / recursion workspace (not live together)
     const uint32_t step = (uint32_t)bp.ctl[0];
     const int g = blockIdx.x;
     for (int s = bp.sched_off[g]; s < bp.sched_off[g + 1]; ++s) {
         const int ri = bp.sched[s];
         const RepDesc& r = bp.reps[ri];
-        if (mode == 0) {
-            wg_momenta<D>(r, bp, false, step, 0, red);
+        const bool integ = mode == BM_STEP || mode == BM_INTEGRATOR;
+        if (integ) {
+            wg_momenta<D>(r, bp, false, step, 0, red, lds);
             wg_kick_drift<D>(r, bp);
         }
-        wg_bead_forces<D, KIND>(r, bp);
-        if (r.bosonic) wg_exchange<D>(r, bp, lds, ri);
-        if (mode == 0) wg_momenta<D>(r, bp, true, step + 1, 1, red);
+        if (mode <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
+        if (r.bosonic && (mode <= BM_FORCES || mode == BM_EXCHANGE)) wg_exchange<D>(r, bp, lds, ri);
+        if (integ) wg_momenta<D>(r, bp, true, step + 1, 1, red, lds);
     }
-    if (mode == 0) {
+    if (mode == BM_STEP) {
         // the last workgroup to finish advances the step counter
         __shared__ int last;
         __syncthreads();

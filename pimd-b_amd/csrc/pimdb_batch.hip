@@ -97,14 +97,21 @@ static int upload_schedule(pimdb_batch* b) {
     return PIMDB_OK;
 }
 
+template <int D, int KIND>
+static void launch_batch_dk(pimdb_batch* b, int mode) {
+    dim3 grid(b->grid), block(b->threads);
+    if (b->threads <= 256) hipLaunchKernelGGL((k_batch_step<D, KIND, 256>), grid, block, b->lds, b->s, b->bp, mode);
+    else if (b->threads <= 512) hipLaunchKernelGGL((k_batch_step<D, KIND, 512>), grid, block, b->lds, b->s, b->bp, mode);
+    else hipLaunchKernelGGL((k_batch_step<D, KIND, 1024>), grid, block, b->lds, b->s, b->bp, mode);
+}
+
 template <int D>
 static void launch_batch_d(pimdb_batch* b, int mode) {
-    dim3 grid(b->grid), block(b->threads);
     switch (b->kind) {
-    case POT_HARMONIC: hipLaunchKernelGGL((k_batch_step<D, POT_HARMONIC>), grid, block, b->lds, b->s, b->bp, mode); break;
-    case POT_DIPOLE: hipLaunchKernelGGL((k_batch_step<D, POT_DIPOLE>), grid, block, b->lds, b->s, b->bp, mode); break;
-    case POT_AZIZ: hipLaunchKernelGGL((k_batch_step<D, POT_AZIZ>), grid, block, b->lds, b->s, b->bp, mode); break;
-    default: hipLaunchKernelGGL((k_batch_step<D, POT_FREE>), grid, block, b->lds, b->s, b->bp, mode); break;
+    case POT_HARMONIC: launch_batch_dk<D, POT_HARMONIC>(b, mode); break;
+    case POT_DIPOLE: launch_batch_dk<D, POT_DIPOLE>(b, mode); break;
+    case POT_AZIZ: launch_batch_dk<D, POT_AZIZ>(b, mode); break;
+    default: launch_batch_dk<D, POT_FREE>(b, mode); break;
     }
 }
 
@@ -122,7 +129,7 @@ static int capture(pimdb_batch* b) {
     if (b->gexec) return PIMDB_OK;
     BCK(hipStreamSynchronize(b->s));
     BCK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-    int rc = launch_batch(b, 0);
+    int rc = launch_batch(b, BM_STEP);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->s, &g);
     if (rc) {
@@ -205,9 +212,15 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         b->desc.push_back(d);
         Nmax = std::max(Nmax, p.natoms);
     }
-    b->lds = sizeof(double) * (16 + rec_shared_doubles(Nmax));
+    size_t need = 0;
+    for (int r = 0; r < nrep; ++r) {
+        const pimdb_params& p = reps[r];
+        need = std::max(need, rec_shared_doubles(p.natoms));
+        need = std::max(need, batch_force_lds_doubles(p.natoms, p.nbeads, p.ndim, 1024));
+    }
+    b->lds = sizeof(double) * (16 + need);
     int rc = PIMDB_OK;
-    if (b->lds > device_lds_limit()) rc = b->fail(PIMDB_E_ARG, "natoms too large for the batched exchange");
+    if (b->lds > device_lds_limit()) rc = b->fail(PIMDB_E_ARG, "natoms too large for the batched workgroup");
     auto alloc = [&](double** ptr, size_t n) {
         if (rc) return;
         if (hipMalloc((void**)ptr, std::max<size_t>(1, n) * sizeof(double)) != hipSuccess ||
@@ -215,7 +228,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             rc = b->fail(PIMDB_E_HIP, "device allocation failed");
     };
     double *Ct = nullptr, *CTt = nullptr;
-    alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx); alloc(&b->bp.pnm, b->nx);
+    alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
     alloc(&b->bp.E, b->nE); alloc(&b->bp.ET, b->nE); alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
     alloc(&Ct, b->nC); alloc(&CTt, b->nC);
     b->bp.Ct = Ct;
@@ -259,7 +272,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->s) (void)hipStreamSynchronize(b->s);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.pnm, b->bp.E, b->bp.ET, b->bp.V, b->bp.Vb,
+    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.E, b->bp.ET, b->bp.V, b->bp.Vb,
                       const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
@@ -310,7 +323,7 @@ int pimdb_batch_get_state(pimdb_batch* b, int rep, double* x, double* p, double*
 
 int pimdb_batch_forces(pimdb_batch* b) {
     if (!b) return PIMDB_E_ARG;
-    int rc = launch_batch(b, 1);
+    int rc = launch_batch(b, BM_FORCES);
     if (rc) return rc;
     return check_batch_status(b);
 }
@@ -347,6 +360,44 @@ int pimdb_batch_time_step(pimdb_batch* b, int reps, double* ms_step) {
     float ms = 0.0f;
     BCK(hipEventElapsedTime(&ms, b->t0, b->t1));
     *ms_step = (double)ms / reps;
+    return check_batch_status(b);
+}
+
+// Device ms of one launch of a phase (0 step, 1 pair forces, 2 exchange,
+// 3 integrator), averaged over reps launches, from a saved state that is
+// restored afterwards.
+int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) {
+    if (!b || reps < 1 || !ms_out || phase < 0 || phase > 3) return PIMDB_E_ARG;
+    static const int modes[4] = {BM_STEP, BM_PAIRS, BM_EXCHANGE, BM_INTEGRATOR};
+    double *sx = nullptr, *sp = nullptr, *sf = nullptr;
+    unsigned long long sctl[2];
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipMalloc((void**)&sx, sizeof(double) * b->nx));
+    BCK(hipMalloc((void**)&sp, sizeof(double) * b->nx));
+    BCK(hipMalloc((void**)&sf, sizeof(double) * b->nx));
+    BCK(hipMemcpy(sx, b->bp.x, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
+    int rc = launch_batch(b, modes[phase]);   // warm
+    if (!rc) {
+        BCK(hipEventRecord(b->t0, b->s));
+        for (int r = 0; r < reps && !rc; ++r) rc = launch_batch(b, modes[phase]);
+        BCK(hipEventRecord(b->t1, b->s));
+        BCK(hipEventSynchronize(b->t1));
+        float ms = 0.0f;
+        BCK(hipEventElapsedTime(&ms, b->t0, b->t1));
+        *ms_out = (double)ms / reps;
+    }
+    BCK(hipStreamSynchronize(b->s));
+    BCK(hipMemcpy(b->bp.x, sx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
+    (void)hipFree(sx);
+    (void)hipFree(sp);
+    (void)hipFree(sf);
+    if (rc) return rc;
     return check_batch_status(b);
 }
 

@@ -101,6 +101,7 @@ def lib():
         "pimdb_batch_configure": (i, [vp, i, i]),
         "pimdb_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
         "pimdb_batch_time_step": (i, [vp, i, _DP]),
+        "pimdb_batch_time_phase": (i, [vp, i, i, _DP]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -258,6 +259,12 @@ class Batch:
     def time_step(self, reps=5):
         v = np.zeros(1)
         self._chk(lib().pimdb_batch_time_step(self.h, int(reps), _p(v)))
+        return float(v[0])
+
+    def time_phase(self, phase, reps=3):
+        """phase: 0 step, 1 pair forces, 2 exchange, 3 integrator (ms/launch)."""
+        v = np.zeros(1)
+        self._chk(lib().pimdb_batch_time_phase(self.h, int(phase), int(reps), _p(v)))
         return float(v[0])
 
 
