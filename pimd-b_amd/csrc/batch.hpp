@@ -15,6 +15,7 @@
 
 #include "device_common.hpp"
 #include "exchange.hpp"
+#include "pairs.hpp"
 
 namespace pimdb {
 
@@ -147,99 +148,51 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r, const BatchPtrs& bp
     __syncthreads();
 }
 
-// Doubles of LDS the force phase stages positions in (a group of beads).
-constexpr int BATCH_POS_LDS = 8192;
-
 // Physical forces and interior springs on every (bead, particle) of a replica
-// (simulation.cpp:353-455).  Beads are taken in groups whose positions fit in
-// LDS; work items (bead of the group, particle i) with i fastest, each running
-// a slice of the partners j from LDS; slices are summed in LDS.
+// (simulation.cpp:353-455).  Each wave owns whole beads: the pairs of a bead
+// with Newton's third law (slice_pair_forces), then the trap and the springs.
 template <int D, int KIND>
-__device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& bp, double* lds) {
+__device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& bp) {
     const int N = r.N, P = r.P;
     const double* x = bp.x + r.off_x;
     double* f = bp.f + r.off_x;
     const bool pbc = r.pbc != 0;
     const size_t bs = (size_t)D * N;
-    const int G = max(1, min(P, BATCH_POS_LDS / (D * N)));
-    double* sx = lds;                                // [G][D][N]
-    double* part = lds + (size_t)G * D * N;          // [nslice-1][D][G*N]
-    for (int b0 = 0; b0 < P; b0 += G) {
-        const int g = min(G, P - b0);
-        const int items = g * N;
-        const int nslice = max(1, min((int)blockDim.x / items, N));
-        for (int e = threadIdx.x; e < g * D * N; e += blockDim.x) sx[e] = x[bs * b0 + e];
-        __syncthreads();
-        const int chunk = (N + nslice - 1) / nslice;
-        for (int t = threadIdx.x; t < items * nslice; t += blockDim.x) {
-            const int item = t % items, slice = t / items;
-            const int bl = item / N, i = item % N;
-            const double* xb = sx + (size_t)bl * D * N;
-            double xi[D], acc[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) { xi[a] = xb[a * N + i]; acc[a] = 0.0; }
-            if constexpr (KIND != POT_FREE) {
-                if (r.cutoff != 0.0) {
-                    const int jlo = slice * chunk, jhi = min(N, jlo + chunk);
-                    for (int j = jlo; j < jhi; ++j) {
-                        if (j == i) continue;
-                        double d[D];
-#pragma unroll
-                        for (int a = 0; a < D; ++a) d[a] = disp_fast(xb[a * N + j], xi[a], pbc, r.L, r.invL);
-                        const double r2 = norm2<D>(d);
-                        const double norm = sqrt(r2);
-                        if (norm < r.cutoff || r.cutoff < 0.0) {
-                            const double gg = pair_grad_factor_fast<KIND>(norm, 1.0 / norm, r.int_k, r.strength);
-#pragma unroll
-                            for (int a = 0; a < D; ++a) acc[a] += -(gg * d[a]);
-                        }
-                    }
-                }
-            }
-            if (slice > 0) {
-#pragma unroll
-                for (int a = 0; a < D; ++a) part[((size_t)(slice - 1) * D + a) * items + item] = acc[a];
-            } else {
-#pragma unroll
-                for (int a = 0; a < D; ++a) f[bs * b0 + (size_t)bl * bs + (size_t)a * N + i] = acc[a];
-            }
+    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.int_k, r.strength);
+    for (int b = wave; b < P; b += nw) {
+        const double* xs = x + bs * b;
+        double* fo = f + bs * b;
+        if (KIND != POT_FREE && r.cutoff != 0.0) {
+            slice_pair_forces<D, KIND>(xs, fo, N, pp, lane);
+        } else {
+            for (int e = lane; e < D * N; e += WAVE) fo[e] = 0.0;
+            __threadfence_block();
         }
-        __syncthreads();
-        // sum the slices, add the trap and the interior springs
-        for (int item = threadIdx.x; item < items; item += blockDim.x) {
-            const int bl = item / N, i = item % N, b = b0 + bl;
-            const bool exchange_bead = r.bosonic && (b == 0 || b == P - 1);
-            const double* xprev = x + bs * ((b - 1 + P) % P);
-            const double* xnext = x + bs * ((b + 1) % P);
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const size_t idx = bs * b + (size_t)a * N + i;
-                double acc = f[idx];
-                for (int sl = 1; sl < nslice; ++sl) acc += part[((size_t)(sl - 1) * D + a) * items + item];
-                const double xi = sx[(size_t)bl * D * N + a * N + i];
-                const double ext = r.ext_harmonic ? (xi * r.ext_k) * (-1.0) : -0.0;
-                const double phys = ext + acc;
-                double spring = 0.0;
-                if (!exchange_bead) {
-                    const double dp = disp(xi, xprev[(size_t)a * N + i], pbc, r.L);
-                    const double dn = disp(xi, xnext[(size_t)a * N + i], pbc, r.L);
-                    spring = r.k * (dp + dn);
-                }
-                f[idx] = exchange_bead ? phys : spring + phys;
+        // trap and interior springs; exchange beads get their spring part
+        // from the exchange phase
+        const bool exchange_bead = r.bosonic && (b == 0 || b == P - 1);
+        const double* xprev = x + bs * ((b - 1 + P) % P);
+        const double* xnext = x + bs * ((b + 1) % P);
+        for (int e = lane; e < D * N; e += WAVE) {
+            const double xi = xs[e];
+            const double ext = r.ext_harmonic ? (xi * r.ext_k) * (-1.0) : -0.0;
+            const double phys = ext + fo[e];
+            double spring = 0.0;
+            if (!exchange_bead) {
+                const double dp = disp(xi, xprev[e], pbc, r.L);
+                const double dn = disp(xi, xnext[e], pbc, r.L);
+                spring = r.k * (dp + dn);
             }
+            fo[e] = exchange_bead ? phys : spring + phys;
         }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
-// LDS doubles the force phase needs for a replica of N particles in D axes
-// (positions of a bead group plus the slice partials).
-__host__ __device__ inline size_t batch_force_lds_doubles(int N, int P, int D, int threads) {
-    const int G = BATCH_POS_LDS / (D * N) > 1 ? (BATCH_POS_LDS / (D * N) < P ? BATCH_POS_LDS / (D * N) : P) : 1;
-    const size_t forces = (size_t)G * D * N + (size_t)threads * D;
-    const size_t nm = 2 * (size_t)P * WAVE;   // normal-mode column tiles
-    return forces > nm ? forces : nm;
-}
+// LDS doubles of the integrator's normal-mode column tiles.
+__host__ __device__ inline size_t batch_nm_lds_doubles(int P) { return 2 * (size_t)P * WAVE; }
 
 // Exchange of one replica: cycle energies, both recursions, exterior forces
 // added into f of beads 1 and P.
@@ -290,7 +243,7 @@ template <int D, int KIND, int MAXT>
 static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mode) {
     extern __shared__ double smem[];
     double* red = smem;                 // [16]
-    double* lds = smem + 16;            // force staging / recursion workspace (not live together)
+    double* lds = smem + 16;            // normal-mode tiles / recursion workspace (not live together)
     const uint32_t step = (uint32_t)bp.ctl[0];
     const int g = blockIdx.x;
     for (int s = bp.sched_off[g]; s < bp.sched_off[g + 1]; ++s) {
@@ -301,7 +254,7 @@ static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mo
             wg_momenta<D>(r, bp, false, step, 0, red, lds);
             wg_kick_drift<D>(r, bp);
         }
-        if (mode <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
+        if (mode <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
         if (r.bosonic && (mode <= BM_FORCES || mode == BM_EXCHANGE)) wg_exchange<D>(r, bp, lds, ri);
         if (integ) wg_momenta<D>(r, bp, true, step + 1, 1, red, lds);
     }

@@ -1,0 +1,154 @@
+// pairs.hpp -- pair forces of one time slice on one wave, Newton's third law,
+// in a fixed order.
+//
+// Reference: Simulation::updatePhysicalForces (src/simulation.cpp:428-455):
+// every unordered pair (i<j) within the cutoff adds -gradV(r_i - r_j) to i and
+// its negative to j.
+//
+// Particles are cut into tiles of 64.  For a tile pair (I, J), I <= J, lane l
+// owns row particle I*64+l (position and force in registers) and holds one
+// column particle of tile J, whose position, force accumulator and index move
+// one lane along the wave after every step (a DPP wave rotation), so at every
+// step the 64 lanes meet 64 distinct column particles and no two lanes touch
+// the same accumulator.  An off-diagonal tile pair takes 64 steps (all 64 x 64
+// pairs); a diagonal tile takes offsets 1..32 (offset 32 on lanes < 32 only),
+// each unordered pair of the tile exactly once.  Which way the rotation turns
+// does not matter: the column index travels with the data.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace pimdb {
+
+constexpr int PTILE = 64;
+
+struct PairParams {
+    double L, invL;
+    double cutoff, cut2_lo, cut2_hi;   // c^2 (1 -/+ 1e-12): outside this band r^2 decides alone
+    double int_k, strength;
+    int pbc, guard;                    // guard: exact image choice near L/2 (PBC without a cutoff)
+};
+
+__host__ __device__ inline PairParams make_pair_params(double L, int pbc, double cutoff, double int_k,
+                                                       double strength) {
+    PairParams pp;
+    pp.L = L;
+    pp.invL = 1.0 / L;
+    pp.pbc = pbc;
+    pp.cutoff = cutoff;
+    pp.cut2_lo = cutoff * cutoff * (1.0 - 1e-12);
+    pp.cut2_hi = cutoff * cutoff * (1.0 + 1e-12);
+    pp.int_k = int_k;
+    pp.strength = strength;
+    pp.guard = (pbc && cutoff < 0.0) ? 1 : 0;
+    return pp;
+}
+
+// Minimum image.  With a cutoff <= L/2 a displacement whose image choice is
+// ambiguous (|dx| within rounding of L/2) lies outside the cutoff whichever
+// image is taken, so the guarded exact path is only needed without a cutoff.
+__device__ __forceinline__ double min_image_pair(double dx, const PairParams& pp) {
+#pragma clang fp contract(off)
+    if (!pp.pbc) return dx;
+    if (pp.guard) return min_image_fast(dx, pp.L, pp.invL);
+    return dx - pp.L * floor(dx * pp.invL + 0.5);
+}
+
+// 1/sqrt(r2) to double precision: hardware estimate + two Newton steps
+// (each squares the relative error).
+__device__ __forceinline__ double rsqrt_refined(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+
+// Force on i due to j, -gradV(r_i - r_j), if the pair is inside the cutoff
+// (distance < cutoff, or cutoff < 0); false otherwise.
+template <int D, int KIND>
+__device__ __forceinline__ bool pair_force(const double (&xi)[D], const double (&xj)[D], const PairParams& pp,
+                                           double (&f)[D]) {
+    double d[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) d[a] = min_image_pair(xi[a] - xj[a], pp);
+    const double r2 = norm2<D>(d);
+    bool in = pp.cutoff < 0.0 || r2 < pp.cut2_lo;
+    if (!in && pp.cutoff > 0.0 && r2 < pp.cut2_hi) in = sqrt(r2) < pp.cutoff;   // exact test inside the band
+    if (!in) return false;
+    const double inv = rsqrt_refined(r2);
+    const double g = pair_grad_factor_fast<KIND>(r2 * inv, inv, pp.int_k, pp.strength);
+#pragma unroll
+    for (int a = 0; a < D; ++a) f[a] = -(g * d[a]);
+    return true;
+}
+
+// Lane l receives what lane (l+1) mod 64 held (DPP wave_rol:1).  Call with
+// the whole wave active.
+__device__ __forceinline__ int lane_rotate(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ double lane_rotate(double v) {
+    const int lo = lane_rotate(__double2loint(v));
+    const int hi = lane_rotate(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// All pairs of one slice on one wave.  xs: slice positions [D][N]; fo: the
+// slice's force array [D][N], owned by this wave, overwritten with the pair
+// forces.  Must be called by all 64 lanes together.
+template <int D, int KIND>
+__device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, int N, const PairParams& pp, int lane) {
+    for (int e = lane; e < D * N; e += WAVE) fo[e] = 0.0;
+    __threadfence_block();
+    const int T = (N + PTILE - 1) / PTILE;
+    for (int I = 0; I < T; ++I) {
+        const int i = I * PTILE + lane;
+        const bool row_ok = i < N;
+        double xi[D], racc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = row_ok ? xs[a * N + i] : 0.0;
+            racc[a] = 0.0;
+        }
+        for (int J = I; J < T; ++J) {
+            const bool diag = J == I;
+            const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
+            int jc = J * PTILE + ((lane + s0) & (PTILE - 1));
+            double xj[D], cacc[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                xj[a] = jc < N ? xs[a * N + jc] : 0.0;
+                cacc[a] = 0.0;
+            }
+            for (int s = s0; s < s1; ++s) {
+                if (row_ok && jc < N && (!diag || s < 32 || lane < 32)) {
+                    double f[D];
+                    if (pair_force<D, KIND>(xi, xj, pp, f)) {
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            racc[a] += f[a];
+                            cacc[a] -= f[a];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    xj[a] = lane_rotate(xj[a]);
+                    cacc[a] = lane_rotate(cacc[a]);
+                }
+                jc = lane_rotate(jc);
+            }
+            if (jc < N) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) fo[a * N + jc] += cacc[a];
+            }
+            __threadfence_block();
+        }
+        if (row_ok) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) fo[a * N + i] += racc[a];
+        }
+        __threadfence_block();
+    }
+}
+
+}  // namespace pimdb

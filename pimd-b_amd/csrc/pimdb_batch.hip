@@ -216,7 +216,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     for (int r = 0; r < nrep; ++r) {
         const pimdb_params& p = reps[r];
         need = std::max(need, rec_shared_doubles(p.natoms));
-        need = std::max(need, batch_force_lds_doubles(p.natoms, p.nbeads, p.ndim, 1024));
+        need = std::max(need, batch_nm_lds_doubles(p.nbeads));
     }
     b->lds = sizeof(double) * (16 + need);
     int rc = PIMDB_OK;
