@@ -54,96 +54,227 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
     return s;
 }
 
-// Optional half kick, O-step (Cartesian or normal-mode coupling), optional
-// zeroMomentum, for one replica (langevin.cpp:15-27, thermostat_coupling.cpp,
-// simulation.cpp:581-603).
+#ifndef BATCH_REC_UNROLL
+#define BATCH_REC_UNROLL 4
+#endif
+#ifndef BATCH_EXT_UNROLL
+#define BATCH_EXT_UNROLL 1
+#endif
+
+// Wave-uniform copies (v_readfirstlane) of values that reach a non-inlined
+// function in vector registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+    const unsigned long long u = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+// Wave-uniform copy of a replica descriptor.  The phases below are kept out
+// of line (each one's registers would otherwise add to the others'); their
+// arguments arrive in vector registers, and these copies put the replica's
+// sizes, constants and offsets back into scalar registers.
+__device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
+    RepDesc u;
+    u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
+    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.ext_harmonic = uni(r.ext_harmonic);
+    u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
+    u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
+    u.ext_k = uni(r.ext_k); u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
+    u.seed = (uint32_t)uni((int)r.seed); u.rid = (uint32_t)uni((int)r.rid);
+    u.off_x = (unsigned long long)uni((double*)r.off_x);
+    u.off_E = (unsigned long long)uni((double*)r.off_E);
+    u.off_V = (unsigned long long)uni((double*)r.off_V);
+    u.off_C = (unsigned long long)uni((double*)r.off_C);
+    return u;
+}
+
+// Read-only matrices through the constant address space: wave-uniform reads
+// of it become scalar loads.
+typedef const __attribute__((address_space(4))) double* const_dptr;
+
+// Modes (and beads) per wave in the normal-mode transforms; even, so that the
+// two modes of a Box-Muller pair land on the same lane.
+constexpr int NM_BLOCK = 4;
+
+// p[b][a][*] -= cm[a] for every bead and axis of a replica.
 template <int D>
-__device__ __noinline__ void wg_momenta(const RepDesc& r, const BatchPtrs& bp, bool kick, uint32_t step, uint32_t ostep,
-                                        double* red, double* lds) {
+__device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double (&cm)[D]) {
+    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE, wave = threadIdx.x / WAVE;
+    for (int row = wave; row < P * D; row += nw) {
+        const double c = cm[row % D];
+        for (int i = lane; i < N; i += WAVE) p[(size_t)row * N + i] -= c;
+    }
+}
+
+// Optional half kick, O-step (Cartesian or normal-mode coupling) and the
+// centre-of-mass momentum of the result, for one replica (langevin.cpp:15-27,
+// thermostat_coupling.cpp, simulation.cpp:581-603).  With fixcom the COM
+// momentum per axis is returned in cm and, when `subtract`, removed from p
+// here; otherwise the caller removes it (wg_kick_drift fuses it).
+template <int D>
+__device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, bool kick_, uint32_t step_,
+                                        uint32_t ostep_, double* red_, double* lds_, bool subtract_, double (&cm)[D]) {
+    const RepDesc r = rep_uniform(r_);
+    const bool kick = uni((int)kick_) != 0, subtract = uni((int)subtract_) != 0;
+    const uint32_t step = (uint32_t)uni((int)step_), ostep = (uint32_t)uni((int)ostep_);
+    double* red = uni(red_);
+    double* lds = uni(lds_);
     const int N = r.N, P = r.P;
-    const int tot = P * D * N;
-    double* p = bp.p + r.off_x;
-    const double* f = bp.f + r.off_x;
+    double* p = uni(bp.p) + r.off_x;
+    const double* f = uni(bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
+    const int ncols = D * N;
+    double csum[D];   // this thread's share of the momentum sums per axis
+#pragma unroll
+    for (int a = 0; a < D; ++a) csum[a] = 0.0;
     if (r.thermostat != 1 || !r.nm) {
-        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-            double v = p[e];
-            if (kick) v += half_dt * f[e];
-            if (r.thermostat == 1) {
-                double g0, g1;
-                gauss2((uint32_t)e, step, ostep, r.rid, r.seed, g0, g1);
-                v = r.c1 * v + r.c2 * g0;
+        // beads 2m and 2m+1 of a column share one Box-Muller pair
+        const int half = (P + 1) / 2;
+        for (int e2 = threadIdx.x; e2 < half * ncols; e2 += blockDim.x) {
+            const int m = e2 / ncols, col = e2 % ncols;
+            const int ax = col / N;
+            double g[2] = {0.0, 0.0};
+            if (r.thermostat == 1) gauss2((uint32_t)e2, step, ostep, r.rid, r.seed, g[0], g[1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int b = 2 * m + h;
+                if (b < P) {
+                    const size_t e = (size_t)b * ncols + col;
+                    double v = p[e];
+                    if (kick) v += half_dt * f[e];
+                    if (r.thermostat == 1) v = r.c1 * v + r.c2 * g[h];
+                    p[e] = v;
+#pragma unroll
+                    for (int a = 0; a < D; ++a)
+                        if (a == ax) csum[a] += v;
+                }
             }
-            p[e] = v;
         }
     } else {
-        if (kick) {
-            for (int e = threadIdx.x; e < tot; e += blockDim.x) p[e] += half_dt * f[e];
-            __syncthreads();
-        }
-        const double* Ct = bp.Ct + r.off_C;
-        const double* CTt = bp.CTt + r.off_C;
-        // Columns (a, i) in tiles of 64 staged in LDS; wave w computes modes
-        // k = w, w + nwave, ... for the 64 columns of the tile (lane = column),
-        // so C[k][j] is the same for the whole wave.
-        const int ncols = D * N;
-        const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+        // Ct[j*P + k] = C[k][j], CTt[k*P + b] = CT[b][k]; both padded by
+        // NM_BLOCK doubles so a block may read past the last row.
+        const const_dptr Ct = (const_dptr)(uni(bp.Ct) + r.off_C);
+        const const_dptr CTt = (const_dptr)(uni(bp.CTt) + r.off_C);
+        // Columns (a, i) in tiles of 64 staged in LDS (after the half kick),
+        // lane = column.  Wave w computes modes (then beads) 4w .. 4w+3,
+        // 4(w+nw) .. of its column, so one LDS read feeds four FMAs and the
+        // matrix entries are the same for the whole wave.
+        const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+        const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
         double* pc = lds;                        // [P][64]
         double* pn = lds + (size_t)P * WAVE;     // [P][64]
         for (int c0 = 0; c0 < ncols; c0 += WAVE) {
             const int col = c0 + lane;
             const bool valid = col < ncols;
+            const int ax = col / N;
             for (int e = threadIdx.x; e < P * WAVE; e += blockDim.x) {
                 const int j = e / WAVE, c = c0 + e % WAVE;
-                pc[e] = c < ncols ? p[(size_t)j * ncols + c] : 0.0;
+                double v = 0.0;
+                if (c < ncols) {
+                    const size_t g = (size_t)j * ncols + c;
+                    v = p[g];
+                    if (kick) v += half_dt * f[g];
+                }
+                pc[e] = v;
             }
             __syncthreads();
-            // mode k of column (a, i): sum_j C[k][j] p[j][a][i], then the Langevin update
-            for (int k = wave; k < P; k += nw) {
-                double s = 0.0;
-                for (int j = 0; j < P; ++j) s += Ct[(size_t)j * P + k] * pc[j * WAVE + lane];
-                double g0 = 0.0, g1;
-                if (valid) gauss2((uint32_t)((size_t)k * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
-                pn[k * WAVE + lane] = r.c1 * s + r.c2 * g0;
+            // modes: sum_j C[k][j] p_j, then the Langevin update; modes 2m
+            // and 2m+1 share one Box-Muller pair
+            for (int kb = NM_BLOCK * wave; kb < P; kb += NM_BLOCK * nw) {
+                double sm[NM_BLOCK];
+#pragma unroll
+                for (int q = 0; q < NM_BLOCK; ++q) sm[q] = 0.0;
+#pragma unroll 4
+                for (int j = 0; j < P; ++j) {
+                    const double v = pc[j * WAVE + lane];
+                    const const_dptr cr = Ct + (size_t)j * P + kb;
+#pragma unroll
+                    for (int q = 0; q < NM_BLOCK; ++q) sm[q] += cr[q] * v;
+                }
+#pragma unroll
+                for (int h = 0; h < NM_BLOCK; h += 2) {
+                    const int k = kb + h;
+                    if (k < P) {
+                        double g0 = 0.0, g1 = 0.0;
+                        if (valid) gauss2((uint32_t)((size_t)(k >> 1) * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                        pn[k * WAVE + lane] = r.c1 * sm[h] + r.c2 * g0;
+                        if (k + 1 < P) pn[(k + 1) * WAVE + lane] = r.c1 * sm[h + 1] + r.c2 * g1;
+                    }
+                }
             }
             __syncthreads();
-            for (int b = wave; b < P; b += nw) {
-                double s = 0.0;
-                for (int k = 0; k < P; ++k) s += CTt[(size_t)k * P + b] * pn[k * WAVE + lane];
-                if (valid) p[(size_t)b * ncols + col] = s;
+            for (int bb = NM_BLOCK * wave; bb < P; bb += NM_BLOCK * nw) {
+                double sm[NM_BLOCK];
+#pragma unroll
+                for (int q = 0; q < NM_BLOCK; ++q) sm[q] = 0.0;
+#pragma unroll 4
+                for (int k = 0; k < P; ++k) {
+                    const double v = pn[k * WAVE + lane];
+                    const const_dptr cr = CTt + (size_t)k * P + bb;
+#pragma unroll
+                    for (int q = 0; q < NM_BLOCK; ++q) sm[q] += cr[q] * v;
+                }
+                if (valid) {
+#pragma unroll
+                    for (int q = 0; q < NM_BLOCK; ++q) {
+                        if (bb + q < P) {
+                            p[(size_t)(bb + q) * ncols + col] = sm[q];
+#pragma unroll
+                            for (int a = 0; a < D; ++a)
+                                if (a == ax) csum[a] += sm[q];
+                        }
+                    }
+                }
             }
             __syncthreads();
         }
     }
     if (r.fixcom) {
-        __syncthreads();
-        double cm[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            double s = 0.0;
-            for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
-                const int b = e / N, i = e % N;
-                s += p[((size_t)b * D + a) * N + i];
-            }
-            cm[a] = wg_sum(s, red) / (double)(N * P);
+        for (int a = 0; a < D; ++a) cm[a] = wg_sum(csum[a], red) / (double)(N * P);
+        if (subtract) {
+            __syncthreads();
+            wg_sub_com<D>(p, N, P, cm);
         }
-        for (int e = threadIdx.x; e < tot; e += blockDim.x) p[e] -= cm[(e / N) % D];
+    } else {
+#pragma unroll
+        for (int a = 0; a < D; ++a) cm[a] = 0.0;
     }
     __syncthreads();
 }
 
-// B + A: p += dt/2 f, x += dt p / m (velocity_verlet.cpp:24-38)
+// zeroMomentum (cm, when fixcom), then B + A: p += dt/2 f, x += dt p / m
+// (simulation.cpp:581-603, velocity_verlet.cpp:24-38).
 template <int D>
-__device__ __noinline__ void wg_kick_drift(const RepDesc& r, const BatchPtrs& bp) {
-    const int tot = r.P * D * r.N;
-    double* x = bp.x + r.off_x;
-    double* p = bp.p + r.off_x;
-    const double* f = bp.f + r.off_x;
+__device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& bp, const double (&cm_)[D]) {
+    const RepDesc r = rep_uniform(r_);
+    double cm[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) cm[a] = uni(cm_[a]);
+    const int N = r.N, P = r.P;
+    double* x = uni(bp.x) + r.off_x;
+    double* p = uni(bp.p) + r.off_x;
+    const double* f = uni(bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const double pv = p[e] + half_dt * f[e];
-        p[e] = pv;
-        x[e] += r.dt * pv / r.mass;
+    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE, wave = threadIdx.x / WAVE;
+    for (int row = wave; row < P * D; row += nw) {
+        const double c = cm[row % D];
+        for (int i = lane; i < N; i += WAVE) {
+            const size_t e = (size_t)row * N + i;
+            double pv = p[e];
+            if (r.fixcom) pv -= c;
+            pv = pv + half_dt * f[e];
+            p[e] = pv;
+            x[e] += r.dt * pv / r.mass;
+        }
     }
     __syncthreads();
 }
@@ -152,10 +283,11 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r, const BatchPtrs& bp
 // (simulation.cpp:353-455).  Each wave owns whole beads: the pairs of a bead
 // with Newton's third law (slice_pair_forces), then the trap and the springs.
 template <int D, int KIND>
-__device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& bp) {
+__device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp) {
+    const RepDesc r = rep_uniform(r_);
     const int N = r.N, P = r.P;
-    const double* x = bp.x + r.off_x;
-    double* f = bp.f + r.off_x;
+    const double* x = uni(bp.x) + r.off_x;
+    double* f = uni(bp.f) + r.off_x;
     const bool pbc = r.pbc != 0;
     const size_t bs = (size_t)D * N;
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
@@ -195,32 +327,41 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r, const BatchPtrs& b
 __host__ __device__ inline size_t batch_nm_lds_doubles(int P) { return 2 * (size_t)P * WAVE; }
 
 // Exchange of one replica: cycle energies, both recursions, exterior forces
-// added into f of beads 1 and P.
+// added into f of beads 1 and P.  Kept out of line (its registers would
+// otherwise add to every other phase's); the arguments are made wave-uniform
+// on entry.
 template <int D>
-__device__ __noinline__ void wg_exchange(const RepDesc& r, const BatchPtrs& bp, double* lds, int ri) {
-    const int N = r.N, P = r.P;
-    const double* x = bp.x + r.off_x;
-    double* f = bp.f + r.off_x;
-    double* E = bp.E + r.off_E;
-    double* ET = bp.ET + r.off_E;
-    double* V = bp.V + r.off_V;
-    double* Vb = bp.Vb + r.off_V;
+__device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
+                                         double* f_, double* E_, double* ET_, double* V_, double* Vb_, int* status_,
+                                         double* lds_) {
+    const int N = uni(N_), P = uni(P_);
+    const bool pbc = uni(pbc_) != 0;
+    const double L = uni(L_), k = uni(k_), bt = uni(bt_);
+    const double* x = uni(x_);
+    double* f = uni(f_);
+    double* E = uni(E_);
+    double* ET = uni(ET_);
+    double* V = uni(V_);
+    double* Vb = uni(Vb_);
+    int* status = uni(status_);
+    double* lds = uni(lds_);
     const size_t bs = (size_t)D * N;
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+    const int wave = uni((int)(threadIdx.x / WAVE));
     const double* x1 = x;
     const double* xP = x + bs * (P - 1);
-    for (int v = wave; v < N; v += nw) cycle_energy_column<D>(v, N, lane, x1, xP, r.k, r.pbc != 0, r.L, E, ET);
+    for (int v = wave; v < N; v += nw) cycle_energy_column<D>(v, N, lane, x1, xP, k, pbc, L, E, ET);
     __threadfence_block();
     __syncthreads();
-    exchange_recursion<0>(N, r.bt, E, ET, V, Vb, bp.status + ri, rec_carve(lds, N));
-    exchange_recursion<1>(N, r.bt, E, ET, V, Vb, bp.status + ri, rec_carve(lds, N));
+    exchange_recursion<0, BATCH_REC_UNROLL>(N, bt, E, ET, V, Vb, status, rec_carve(lds, N));
+    exchange_recursion<1, BATCH_REC_UNROLL>(N, bt, E, ET, V, Vb, status, rec_carve(lds, N));
     __threadfence_block();
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
     for (int task = wave; task < 2 * N; task += nw) {
         const int which = task / N, l = task % N;
         double F[D];
-        exterior_force<D>(which, l, N, lane, r.bt, r.k, r.pbc != 0, r.L, E, ET, V, Vb, x1, x2, xPm1, xP, false, F);
+        exterior_force<D, BATCH_EXT_UNROLL>(which, l, N, lane, bt, k, pbc, L, E, ET, V, Vb, x1, x2, xPm1, xP, false, F);
         if (lane == 0) {
             const size_t b = which == 0 ? 0 : (size_t)(P - 1);
 #pragma unroll
@@ -251,12 +392,18 @@ static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mo
         const RepDesc& r = bp.reps[ri];
         const bool integ = mode == BM_STEP || mode == BM_INTEGRATOR;
         if (integ) {
-            wg_momenta<D>(r, bp, false, step, 0, red, lds);
-            wg_kick_drift<D>(r, bp);
+            double cm[D];
+            wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm);
+            wg_kick_drift<D>(r, bp, cm);
         }
         if (mode <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
-        if (r.bosonic && (mode <= BM_FORCES || mode == BM_EXCHANGE)) wg_exchange<D>(r, bp, lds, ri);
-        if (integ) wg_momenta<D>(r, bp, true, step + 1, 1, red, lds);
+        if (r.bosonic && (mode <= BM_FORCES || mode == BM_EXCHANGE))
+            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.E + r.off_E,
+                           bp.ET + r.off_E, bp.V + r.off_V, bp.Vb + r.off_V, bp.status + ri, lds);
+        if (integ) {
+            double cm[D];
+            wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm);
+        }
     }
     if (mode == BM_STEP) {
         // the last workgroup to finish advances the step counter

@@ -136,11 +136,11 @@ __device__ __forceinline__ double rec_w(int N, const double* __restrict__ E, con
 // One recursion (DIR 0: forward V, DIR 1: backward Vb) for one ring, run by
 // the whole workgroup (64..1024 threads).  Lane t of every wave serves target
 // n0+t of the current block; the waves split the sources of earlier blocks
-// and keep REC_UNROLL loads in flight.  Writes V[0..N] and Vb[0] (DIR 0) or
+// and keep UNROLL loads in flight.  Writes V[0..N] and Vb[0] (DIR 0) or
 // Vb[1..N] (DIR 1); sets *status to 1 when a potential is not finite (the
 // reference throws overflow_error, quadratic_bosonic_exchange.cpp:92-97,
 // 119-124).
-template <int DIR>
+template <int DIR, int UNROLL = REC_UNROLL>
 __device__ void exchange_recursion(int N, double bt, const double* __restrict__ E, const double* __restrict__ ET,
                                    double* __restrict__ V, double* __restrict__ Vb, int* __restrict__ status,
                                    RecShared sh) {
@@ -168,12 +168,12 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
             double m = INFINITY;
             if (active) {
                 int q = qlo;
-                for (; q + REC_UNROLL <= qhi; q += REC_UNROLL) {
-                    double w[REC_UNROLL];
+                for (; q + UNROLL <= qhi; q += UNROLL) {
+                    double w[UNROLL];
 #pragma unroll
-                    for (int r = 0; r < REC_UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+                    for (int r = 0; r < UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
 #pragma unroll
-                    for (int r = 0; r < REC_UNROLL; ++r) m = fmin(m, w[r] + sh.U[q + r]);
+                    for (int r = 0; r < UNROLL; ++r) m = fmin(m, w[r] + sh.U[q + r]);
                 }
                 for (; q < qhi; ++q) m = fmin(m, rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q]);
             }
@@ -202,12 +202,12 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
             if (active) {
                 const double Rn = sh.R[lane];
                 int q = qlo;
-                for (; q + REC_UNROLL <= qhi; q += REC_UNROLL) {
-                    double w[REC_UNROLL];
+                for (; q + UNROLL <= qhi; q += UNROLL) {
+                    double w[UNROLL];
 #pragma unroll
-                    for (int r = 0; r < REC_UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+                    for (int r = 0; r < UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
 #pragma unroll
-                    for (int r = 0; r < REC_UNROLL; ++r) s += exp(-(w[r] + sh.U[q + r] - Rn));
+                    for (int r = 0; r < UNROLL; ++r) s += exp(-(w[r] + sh.U[q + r] - Rn));
                 }
                 for (; q < qhi; ++q) s += exp(-(rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q] - Rn));
             }
@@ -289,10 +289,10 @@ static __global__ void __launch_bounds__(256) k_exchange_probabilities(int N, do
 // (which=1), one wave per (which, l):
 //  bead 1: F[l] = k [ sum_{j>=max(0,l-1)} P[j][l] (xP[j]-x1[l]) + (x2[l]-x1[l]) ]   (:188-215)
 //  bead P: F[l] = k [ sum_{j<=min(l+1,N-1)} P[l][j] (x1[j]-xP[l]) + (xPm1[l]-xP[l]) ] (:159-186)
-// Probabilities are recomputed from V, Vb and E.  With grad_only the
+// Probabilities are recomputed from V, Vb and E (ET required).  With grad_only the
 // interior link is left out and the result is -dV_B/dx (the adjoint of V_B
 // alone, used by the autograd harness).
-template <int D>
+template <int D, int JUNROLL = 4>
 __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane, double bt, double k, bool pbc,
                                                double L, const double* __restrict__ E, const double* __restrict__ ET,
                                                const double* __restrict__ V,
@@ -309,9 +309,21 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
     for (int a = 0; a < D; ++a) S[a] = 0.0;
     const int jlo = which == 0 ? max(0, l - 1) : 0;
     const int jhi = which == 0 ? N - 1 : min(l + 1, N - 1);
+    // P[row][col] (conn_prob's expression) with (row, col) = (j, l) for bead
+    // 1 and (l, j) for bead P; the cycle energy ecyc(col, row) it needs is
+    // ebase[j] in both cases (row l of ET, or row l of the packed E).
+    const double* ebase = which == 0 ? ET + et_row(l, N) - l : E + (size_t)l * (l + 1) / 2;
+    const double VN = V[N];
+#pragma unroll JUNROLL
     for (int j = jlo + lane; j <= jhi; j += WAVE) {
-        // bead 1 walks column l of P (row l of ET), bead P walks row l (column l of E)
-        const double pr = which == 0 ? conn_prob(N, bt, E, ET, V, Vb, j, l) : conn_prob(N, bt, E, nullptr, V, Vb, l, j);
+        const int row = which == 0 ? j : l, col = which == 0 ? l : j;
+        const bool link = col == row + 1 && row < N - 1;
+        double pr = 0.0;
+        if (link || col <= row) {
+            const double arg = link ? V[row + 1] + Vb[row + 1] - VN : V[col] + ebase[j] + Vb[row + 1] - VN;
+            const double ex = exp(-bt * arg);
+            pr = link ? 1.0 - ex : 1.0 / (row + 1) * ex;
+        }
 #pragma unroll
         for (int a = 0; a < D; ++a) S[a] += pr * disp(xs[a], other[a * N + j], pbc, L);
     }

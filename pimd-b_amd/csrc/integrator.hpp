@@ -51,12 +51,17 @@ __device__ __forceinline__ void block_com(const double* part, int npart, int D, 
     __syncthreads();
 }
 
-__device__ __forceinline__ double noise_at(const MomentaArgs& m, int b, int a, int i) {
-    const size_t e = ((size_t)b * m.D + a) * m.N + i;
-    if (m.noise) return m.noise[e];
-    double g0, g1;
-    gauss2((uint32_t)e, (uint32_t)*m.step_ctr, m.ostep, m.replica, m.seed, g0, g1);
-    return g0;
+// Langevin noise of elements (2m, a, i) and (2m+1, a, i) (beads, or modes of
+// the normal-mode coupling): host noise when given, else the two outputs of
+// one Philox/Box-Muller block keyed by (m, a, i).
+__device__ __forceinline__ void noise_pair(const MomentaArgs& m, int mp, int a, int i, double& g0, double& g1) {
+    if (m.noise) {
+        const size_t e = ((size_t)(2 * mp) * m.D + a) * m.N + i;
+        g0 = m.noise[e];
+        g1 = (2 * mp + 1 < m.P) ? m.noise[e + (size_t)m.D * m.N] : 0.0;
+        return;
+    }
+    gauss2((uint32_t)(((size_t)mp * m.D + a) * m.N + i), (uint32_t)*m.step_ctr, m.ostep, m.replica, m.seed, g0, g1);
 }
 
 // Optional COM subtraction, optional half kick, optional O-step, optional
@@ -86,19 +91,36 @@ static __global__ void __launch_bounds__(256) k_momenta(MomentaArgs m, double* _
         pc[e] = v;
     }
     if (m.thermostat == 1) {
+        const int half = (P + 1) / 2;
         if (!m.nm) {
-            for (int e = tid; e < P * TC; e += blockDim.x) {
-                const int b = e / TC, c = e % TC;
-                if (c < ncol) pc[e] = m.c1 * pc[e] + m.c2 * noise_at(m, b, a, i0 + c);
+            for (int e = tid; e < half * TC; e += blockDim.x) {
+                const int mp = e / TC, c = e % TC;
+                if (c < ncol) {
+                    double g[2];
+                    noise_pair(m, mp, a, i0 + c, g[0], g[1]);
+                    for (int h = 0; h < 2 && 2 * mp + h < P; ++h) {
+                        double& v = pc[(2 * mp + h) * TC + c];
+                        v = m.c1 * v + m.c2 * g[h];
+                    }
+                }
             }
         } else {
+            __syncthreads();
+            // noise of modes 2mp, 2mp+1 (one Box-Muller pair) into pn
+            for (int e = tid; e < half * TC; e += blockDim.x) {
+                const int mp = e / TC, c = e % TC;
+                double g0 = 0.0, g1 = 0.0;
+                if (c < ncol) noise_pair(m, mp, a, i0 + c, g0, g1);
+                pn[2 * mp * TC + c] = g0;
+                if (2 * mp + 1 < P) pn[(2 * mp + 1) * TC + c] = g1;
+            }
             __syncthreads();
             // mode k of column c: sum_j C[k][j] p_j, then the Langevin update
             for (int e = tid; e < P * TC; e += blockDim.x) {
                 const int k = e / TC, c = e % TC;
                 double s = 0.0;
                 for (int j = 0; j < P; ++j) s += m.Ct[(size_t)j * P + k] * pc[j * TC + c];
-                pn[e] = (c < ncol) ? m.c1 * s + m.c2 * noise_at(m, k, a, i0 + c) : 0.0;
+                pn[e] = (c < ncol) ? m.c1 * s + m.c2 * pn[e] : 0.0;
             }
             __syncthreads();
             for (int e = tid; e < P * TC; e += blockDim.x) {

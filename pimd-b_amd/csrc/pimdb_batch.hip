@@ -230,7 +230,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     double *Ct = nullptr, *CTt = nullptr;
     alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
     alloc(&b->bp.E, b->nE); alloc(&b->bp.ET, b->nE); alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
-    alloc(&Ct, b->nC); alloc(&CTt, b->nC);
+    alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
     b->bp.Ct = Ct;
     b->bp.CTt = CTt;
     if (!rc && (hipMalloc((void**)&b->bp.status, sizeof(int) * nrep) != hipSuccess ||
