@@ -115,9 +115,11 @@ def bench_plain(kw, x, p, steps, warmup):
     return (t1 - t0) / steps * 1e3, ms_forces, ms_exch, ms_step
 
 
-def bench_batched(kw, x, p, replicas, steps, warmup):
+def bench_batched(kw, x, p, replicas, steps, warmup, threads=0, resident=0, phases=True):
     """R copies of the workload (distinct noise seeds) in the persistent
-    kernel; returns wall ms per batched step, device ms per launch, tuning."""
+    kernel.  Returns wall ms per batched step over the timed region, device ms
+    per launch from stream events around the same region, the launch shape,
+    finiteness and (optionally) per-phase device times."""
     params = []
     for r in range(replicas):
         k = dict(kw)
@@ -127,17 +129,41 @@ def bench_batched(kw, x, p, replicas, steps, warmup):
     for r in range(replicas):
         b.set_state(r, x, p)
     b.forces()
-    threads, resident = b.autotune(trial_steps=2)
+    if threads or resident:
+        b.configure(threads, resident)
+        tuned = (threads, resident)
+    else:
+        tuned = b.autotune(trial_steps=2)
     b.step(warmup)
     t0 = time.perf_counter()
-    b.step(steps)            # returns after the stream is synchronised
+    dev_ms = b.step_timed(steps)     # returns after the closing event completed
     t1 = time.perf_counter()
-    ms_launch = b.time_step(reps=5)
-    phases = dict(pairs=b.time_phase(1), exchange=b.time_phase(2), integrator=b.time_phase(3))
+    ph = None
+    if phases:
+        ph = dict(pairs=b.time_phase(1), exchange=b.time_phase(2), integrator=b.time_phase(3))
     xs, _, fs = b.state(0)
     ok = bool(np.all(np.isfinite(xs)) and np.all(np.isfinite(fs)))
     b.close()
-    return (t1 - t0) / steps * 1e3, ms_launch, threads, resident, ok, phases
+    return (t1 - t0) / steps * 1e3, dev_ms / steps, tuned[0], tuned[1], ok, ph
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same
+    counter's reading on a copy kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", "k_batch_step_aziz_n256_p64_r256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(kw, x, p, cpu_steps):
@@ -180,8 +206,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=2000)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threads", type=int, default=0, help="batch workgroup size (0: autotune)")
+    ap.add_argument("--resident", type=int, default=0, help="batch workgroups (0: autotune)")
+    ap.add_argument("--no-phases", action="store_true", help="skip the per-phase timing launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs a ring polymer (or a set of "
@@ -195,7 +224,8 @@ def main():
 
     ms_plain, ms_forces, ms_exch, ms_step_graph = bench_plain(kw, x, p, args.plain_steps, 50)
     R = args.replicas
-    ms_batch, ms_launch, threads, resident, finite, phases = bench_batched(kw, x, p, R, args.steps, args.warmup)
+    ms_batch, ms_launch, threads, resident, finite, phases = bench_batched(
+        kw, x, p, R, args.steps, args.warmup, args.threads, args.resident, not args.no_phases)
     value = R * 1e3 / ms_batch
 
     # dominant kernel: the persistent step kernel (one launch = one MD step of all R replicas)
@@ -203,20 +233,25 @@ def main():
     xp = R * fm["step_exps"]
     achieved = fl / (ms_launch * 1e-3) / 1e12
     t_roof = fl / (PEAK_FP64_TFLOPS * 1e12) + xp / (exp_g * 1e9)
-    roofline = dict(bound="fp64", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                    kernel="k_batch_step<3,AZIZ> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
+    roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic_from_profile(),
+                    compute="FP64 on the VALU: MI355X's FP64 vector peak equals its FP64 matrix peak "
+                            "(78.6 TFLOP/s); the kernel issues no MFMA instruction",
+                    kernel="k_batch_step<3,AZIZ,STEP> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
+                    timing="hipEvents on the batch stream around the timed steps",
                     flops_per_launch=fl, exps_per_launch=xp, frac_with_exps=t_roof / (ms_launch * 1e-3))
     achieved_f = fm["force_kernel_flops"] / (ms_forces * 1e-3) / 1e12
 
     def frac(flops, exps, ms):
         return (flops / (PEAK_FP64_TFLOPS * 1e12) + exps / (exp_g * 1e9)) / (ms * 1e-3)
 
-    batch_phases = dict(
-        ms=phases,
-        pairs_frac=frac(R * fm["force_kernel_flops"], R * fm["force_kernel_exps"], phases["pairs"]),
-        exchange_frac=frac(R * fm["exchange_flops"], R * fm["exchange_exps"], phases["exchange"]),
-        exchange_tflops=R * fm["exchange_flops"] / (phases["exchange"] * 1e-3) / 1e12)
+    batch_phases = None
+    if phases:
+        batch_phases = dict(
+            ms=phases,
+            pairs_frac=frac(R * fm["force_kernel_flops"], R * fm["force_kernel_exps"], phases["pairs"]),
+            exchange_frac=frac(R * fm["exchange_flops"], R * fm["exchange_exps"], phases["exchange"]),
+            exchange_tflops=R * fm["exchange_flops"] / (phases["exchange"] * 1e-3) / 1e12)
 
     out = dict(
         metric="MD steps/s N=256 P=64 Aziz; batched/persistent vs plain GPU @force err<=1e-10",

@@ -166,6 +166,11 @@ int pimdb_batch_configure(pimdb_batch* batch, int threads, int resident);
 /* Times trial_steps steps for each workgroup size and 1 or 2 workgroups per
  * compute unit, keeps the fastest and restores the state it started from. */
 int pimdb_batch_autotune(pimdb_batch* batch, int trial_steps, int* threads_out, int* resident_out);
+/* nsteps MD steps of every replica, as pimdb_batch_step, with the device
+ * time between events recorded on the batch's stream just before the first
+ * and just after the last step returned in device_ms (no reference
+ * counterpart: measurement). */
+int pimdb_batch_step_timed(pimdb_batch* batch, int nsteps, double* device_ms);
 /* Average device time of one batched MD step over reps graph replays
  * (advances the state by reps + 1 steps). */
 int pimdb_batch_time_step(pimdb_batch* batch, int reps, double* ms_step);

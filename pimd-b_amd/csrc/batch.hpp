@@ -378,10 +378,10 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
 enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3, BM_INTEGRATOR = 4 };
 
 // One MD step of every replica (BM_STEP), the forces only (BM_FORCES), or one
-// phase for timing.  MAXT is the largest workgroup size the instantiation is
-// compiled for (it sets the register budget).
-template <int D, int KIND, int MAXT>
-static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mode) {
+// phase for timing.  The mode is a template parameter so that every mode is
+// its own kernel (and its own line in a kernel trace).
+template <int D, int KIND, int MODE>
+static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
     extern __shared__ double smem[];
     double* red = smem;                 // [16]
     double* lds = smem + 16;            // normal-mode tiles / recursion workspace (not live together)
@@ -390,22 +390,22 @@ static __global__ void __launch_bounds__(MAXT) k_batch_step(BatchPtrs bp, int mo
     for (int s = bp.sched_off[g]; s < bp.sched_off[g + 1]; ++s) {
         const int ri = bp.sched[s];
         const RepDesc& r = bp.reps[ri];
-        const bool integ = mode == BM_STEP || mode == BM_INTEGRATOR;
-        if (integ) {
+        constexpr bool integ = MODE == BM_STEP || MODE == BM_INTEGRATOR;
+        if constexpr (integ) {
             double cm[D];
             wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm);
             wg_kick_drift<D>(r, bp, cm);
         }
-        if (mode <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
-        if (r.bosonic && (mode <= BM_FORCES || mode == BM_EXCHANGE))
+        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
+        if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.E + r.off_E,
                            bp.ET + r.off_E, bp.V + r.off_V, bp.Vb + r.off_V, bp.status + ri, lds);
-        if (integ) {
+        if constexpr (integ) {
             double cm[D];
             wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm);
         }
     }
-    if (mode == BM_STEP) {
+    if constexpr (MODE == BM_STEP) {
         // the last workgroup to finish advances the step counter
         __shared__ int last;
         __syncthreads();

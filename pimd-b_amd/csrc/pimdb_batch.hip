@@ -100,9 +100,13 @@ static int upload_schedule(pimdb_batch* b) {
 template <int D, int KIND>
 static void launch_batch_dk(pimdb_batch* b, int mode) {
     dim3 grid(b->grid), block(b->threads);
-    if (b->threads <= 256) hipLaunchKernelGGL((k_batch_step<D, KIND, 256>), grid, block, b->lds, b->s, b->bp, mode);
-    else if (b->threads <= 512) hipLaunchKernelGGL((k_batch_step<D, KIND, 512>), grid, block, b->lds, b->s, b->bp, mode);
-    else hipLaunchKernelGGL((k_batch_step<D, KIND, 1024>), grid, block, b->lds, b->s, b->bp, mode);
+    switch (mode) {
+    case BM_STEP: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_STEP>), grid, block, b->lds, b->s, b->bp); break;
+    case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES>), grid, block, b->lds, b->s, b->bp); break;
+    case BM_PAIRS: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_PAIRS>), grid, block, b->lds, b->s, b->bp); break;
+    case BM_EXCHANGE: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_EXCHANGE>), grid, block, b->lds, b->s, b->bp); break;
+    default: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_INTEGRATOR>), grid, block, b->lds, b->s, b->bp); break;
+    }
 }
 
 template <int D>
@@ -333,6 +337,20 @@ int pimdb_batch_step(pimdb_batch* b, int nsteps) {
     int rc = capture(b);
     if (rc) return rc;
     for (int s = 0; s < nsteps; ++s) BCK(hipGraphLaunch(b->gexec, b->s));
+    return check_batch_status(b);
+}
+
+int pimdb_batch_step_timed(pimdb_batch* b, int nsteps, double* device_ms) {
+    if (!b || nsteps < 1 || !device_ms) return PIMDB_E_ARG;
+    int rc = capture(b);
+    if (rc) return rc;
+    BCK(hipEventRecord(b->t0, b->s));
+    for (int s = 0; s < nsteps; ++s) BCK(hipGraphLaunch(b->gexec, b->s));
+    BCK(hipEventRecord(b->t1, b->s));
+    BCK(hipEventSynchronize(b->t1));
+    float ms = 0.0f;
+    BCK(hipEventElapsedTime(&ms, b->t0, b->t1));
+    *device_ms = (double)ms;
     return check_batch_status(b);
 }
 

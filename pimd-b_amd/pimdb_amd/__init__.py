@@ -101,6 +101,7 @@ def lib():
         "pimdb_batch_configure": (i, [vp, i, i]),
         "pimdb_batch_autotune": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
         "pimdb_batch_time_step": (i, [vp, i, _DP]),
+        "pimdb_batch_step_timed": (i, [vp, i, _DP]),
         "pimdb_batch_time_phase": (i, [vp, i, i, _DP]),
     }
     for name, (res, args) in sigs.items():
@@ -255,6 +256,12 @@ class Batch:
         t, r = C.c_int(), C.c_int()
         self._chk(lib().pimdb_batch_autotune(self.h, int(trial_steps), C.byref(t), C.byref(r)))
         return t.value, r.value
+
+    def step_timed(self, nsteps):
+        """nsteps steps; returns the device ms between stream events around them."""
+        v = np.zeros(1)
+        self._chk(lib().pimdb_batch_step_timed(self.h, int(nsteps), _p(v)))
+        return float(v[0])
 
     def time_step(self, reps=5):
         v = np.zeros(1)
