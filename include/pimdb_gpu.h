@@ -107,6 +107,13 @@ int pimdb_step(pimdb_ctx* ctx, int nsteps, const double* host_noise);
  * are 0 unless both potentials are non-free, as in energy.cpp:96-102. */
 int pimdb_observables(pimdb_ctx* ctx, double* out);
 
+/* Exchange estimators from the current positions of beads 1 and P:
+ * primEstimator (already divided by P under the IPI convention),
+ * getDistinctProbability and getLongestProbability
+ * (quadratic_bosonic_exchange.cpp:222-279).  Any output may be NULL;
+ * PIMDB_E_STATE for a distinguishable system. */
+int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinct, double* prob_longest);
+
 /* Captures one MD step (device noise) in a hipGraph; later pimdb_step calls
  * with host_noise == NULL replay it.  No reference counterpart (the
  * reference has no graph capture). */

@@ -489,6 +489,26 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
     return PIMDB_OK;
 }
 
+int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinct, double* prob_longest) {
+    if (!ctx) return PIMDB_E_ARG;
+    if (!ctx->dv.bosonic) return ctx->fail(PIMDB_E_STATE, "exchange estimators need a bosonic system");
+    const int N = ctx->N, P = ctx->P;
+    const int nt = (N + FORCE_TILE - 1) / FORCE_TILE;
+    double* out = ctx->obs + 2 * (size_t)P + 3 * (size_t)P * nt;   // [3], as pimdb_observables
+    launch_exchange_forward(ctx, ctx->s0);
+    hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
+                       ctx->dv.bt, ctx->E, ctx->V, out);
+    CK(hipGetLastError());
+    int rc = check_status(ctx);
+    if (rc) return rc;
+    double h[3];
+    CK(hipMemcpy(h, out, sizeof(h), hipMemcpyDeviceToHost));
+    if (prim) *prim = h[0];
+    if (prob_distinct) *prob_distinct = h[1];
+    if (prob_longest) *prob_longest = h[2];
+    return PIMDB_OK;
+}
+
 }  // extern "C"
 
 // Device time per repetition of `body`, from `reps` repetitions captured in

@@ -84,6 +84,7 @@ def lib():
         "pimdb_exchange_adjoint": (i, [vp, _DP, _DP]),
         "pimdb_step": (i, [vp, i, _DP]),
         "pimdb_observables": (i, [vp, _DP]),
+        "pimdb_exchange_estimators": (i, [vp, _DP, _DP, _DP]),
         "pimdb_graph_capture": (i, [vp]),
         "pimdb_time_kernels": (i, [vp, i, _DP, _DP, _DP]),
         "pimdb_peak_probe": (i, [_DP, _DP, _DP, C.c_longlong]),
@@ -194,6 +195,12 @@ class Context:
         self._chk(lib().pimdb_observables(self.h, _p(out)))
         return out
 
+
+    def exchange_estimators(self):
+        """(primEstimator, distinct probability, longest probability)."""
+        v = np.zeros(3)
+        self._chk(lib().pimdb_exchange_estimators(self.h, _p(v[0:]), _p(v[1:]), _p(v[2:])))
+        return float(v[0]), float(v[1]), float(v[2])
     def graph_capture(self):
         self._chk(lib().pimdb_graph_capture(self.h))
 

@@ -63,3 +63,36 @@ def test_observables_reference_rows(case):
             ref = row[1 + j]
             assert abs(ob[NAMES.index(name)] - ref) <= 2e-8 * abs(ref) + 1e-8, (name, ob, ref)
     ctx.close()
+
+
+@pytest.mark.parametrize("N,P,replicate", [(8, 4, False), (96, 16, True), (256, 8, False)])
+def test_exchange_estimators(N, P, replicate):
+    """primEstimator / getDistinctProbability / getLongestProbability
+    (quadratic_bosonic_exchange.cpp:222-279) from the GPU entry point against
+    the oracle's restatement on the same positions."""
+    kw, x = aziz_config(N, P, 1.5, replicate=replicate, seed=5)
+    prm = O.make_params(**kw)
+    d = O.derived(prm)
+    E = O.cycle_energies(x[0], x[-1], d["k"], True, kw["size"])
+    ex = O.exchange(x[0], x[1 % P], x[P - 2], x[-1], d["beta_t"], d["k"], True, kw["size"])
+    L = O.lib()
+    prim = L.orc_prim_estimator(N, P, d["beta_t"], O.ptr(E), O.ptr(ex["V"]))
+    pd = L.orc_distinct_probability(N, d["beta_t"], O.ptr(E), O.ptr(ex["V"]))
+    pl = L.orc_longest_probability(N, d["beta_t"], O.ptr(E), O.ptr(ex["V"]))
+    ctx = G.Context(G.make_params(**kw))
+    ctx.set_state(x, np.zeros_like(x))
+    g_prim, g_pd, g_pl = ctx.exchange_estimators()
+    ctx.close()
+    assert abs(g_prim - prim) <= 1e-10 * max(abs(prim), 1e-300), (g_prim, prim)
+    assert abs(g_pd - pd) <= 1e-10 * max(abs(pd), 1e-300) + 1e-300, (g_pd, pd)
+    assert abs(g_pl - pl) <= 1e-10 * max(abs(pl), 1e-300) + 1e-300, (g_pl, pl)
+
+
+def test_exchange_estimators_need_bosons():
+    kw, x = aziz_config(16, 4, 1.5, seed=5)
+    kw["bosonic"] = 0
+    ctx = G.Context(G.make_params(**kw))
+    ctx.set_state(x, np.zeros_like(x))
+    with pytest.raises(G.PimdbError):
+        ctx.exchange_estimators()
+    ctx.close()
