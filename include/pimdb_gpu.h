@@ -34,7 +34,12 @@ enum pimdb_status {
     PIMDB_E_STATE = 4      /* call not valid in the current state             */
 };
 
-enum pimdb_potential { PIMDB_POT_FREE = 0, PIMDB_POT_HARMONIC = 1, PIMDB_POT_DIPOLE = 2, PIMDB_POT_AZIZ = 3 };
+/* Potentials (src/potentials/*.cpp).  DOUBLE_WELL and COSINE are external
+ * potentials only, as in the reference's allowed names (params.cpp:189-190). */
+enum pimdb_potential {
+    PIMDB_POT_FREE = 0, PIMDB_POT_HARMONIC = 1, PIMDB_POT_DIPOLE = 2, PIMDB_POT_AZIZ = 3,
+    PIMDB_POT_DOUBLE_WELL = 4, PIMDB_POT_COSINE = 5
+};
 enum pimdb_propagator { PIMDB_PROP_CARTESIAN = 0, PIMDB_PROP_NORMAL_MODES = 1 };
 enum pimdb_thermostat { PIMDB_THERMO_NONE = 0, PIMDB_THERMO_LANGEVIN = 1 };
 
@@ -48,12 +53,16 @@ typedef struct pimdb_params {
     double dt, gamma;          /* time step, Langevin friction                */
     int bosonic, pbc;          /* bosonic exchange on beads 1 and P; PBC       */
     double size;               /* cubic box length (bohr)                      */
-    int ext_kind;              /* PIMDB_POT_FREE or PIMDB_POT_HARMONIC          */
+    int ext_kind;              /* FREE, HARMONIC, DOUBLE_WELL or COSINE         */
     double ext_omega;          /* trap frequency (energy units, hbar = 1)      */
     int int_kind;              /* pimdb_potential                              */
     double int_omega, int_strength, cutoff; /* cutoff < 0: all pairs           */
     int propagator, thermostat, nmthermostat, fixcom;
     unsigned int seed;         /* device noise stream seed                     */
+    /* external double well: strength (energy), location (length); external
+     * cosine: amplitude (energy), phase (number), wavelength = size
+     * (double_well.cpp:3-4, cosine.cpp:4-7, simulation.cpp:624-637) */
+    double ext_strength, ext_location, ext_amplitude, ext_phase;
 } pimdb_params;
 
 typedef struct pimdb_ctx pimdb_ctx;

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _DP = C.POINTER(C.c_double)
 
-POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3}
+POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
 THERMO = {"none": 0, "langevin": 1}
 
@@ -30,6 +30,8 @@ class Params(C.Structure):
         ("propagator", C.c_int), ("thermostat", C.c_int),
         ("nmthermostat", C.c_int), ("fixcom", C.c_int),
         ("seed", C.c_uint),
+        ("ext_strength", C.c_double), ("ext_location", C.c_double),
+        ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
     ]
 
 
@@ -77,6 +79,10 @@ def _load():
         "orc_dipole_V": (d, [i, d, _DP]),
         "orc_aziz_grad": (v, [i, _DP, _DP]),
         "orc_aziz_V": (d, [i, _DP]),
+        "orc_double_well_grad": (v, [i, d, d, d, _DP, _DP]),
+        "orc_double_well_V": (d, [i, i, d, d, d, _DP]),
+        "orc_cosine_grad": (v, [i, d, d, d, _DP, _DP]),
+        "orc_cosine_V": (d, [i, i, d, d, d, _DP]),
         "orc_effective_cutoff": (d, [C.POINTER(Params)]),
         "orc_physical_forces": (v, [C.POINTER(Params), _DP, _DP]),
         "orc_update_forces": (i, [C.POINTER(Params), _DP, _DP]),

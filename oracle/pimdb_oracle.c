@@ -292,6 +292,44 @@ static double vnorm(int D, const double* x) {   /* VectorArray::norm, common.h:2
     return sqrt(s);
 }
 
+/* double_well.cpp:6-40: V sums (x_a^2 - loc^2)^2 over particles and axes,
+ * times m*strength; gradV is the isotropic form 4 m s (|x|^2 - loc^2) x. */
+void orc_double_well_grad(int D, double mass, double strength, double loc, const double* x, double* g) {
+    const double loc2 = loc * loc;
+    double pref = 0.0;
+    for (int a = 0; a < D; ++a) pref += x[a] * x[a];
+    pref = 4 * mass * strength * (pref - loc2);
+    for (int a = 0; a < D; ++a) g[a] = pref * x[a];
+}
+double orc_double_well_V(int N, int D, double mass, double strength, double loc, const double* coord) {
+    const double loc2 = loc * loc;
+    double pot = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < D; ++a) {
+            const double x = coord[i * D + a];
+            pot += (x * x - loc2) * (x * x - loc2);
+        }
+    pot *= mass * strength;
+    return pot;
+}
+
+/* cosine.cpp:4-36: k = 2 pi / wavelength (the box size, simulation.cpp:634-637);
+ * V = A sum cos(k x + phase); gradV = -A k sin(k x + phase). */
+static const double ORC_PI = 3.14159265358979323846;
+void orc_cosine_grad(int D, double amplitude, double wavelength, double phase, const double* x, double* g) {
+    const double k = 2 * ORC_PI / wavelength;
+    const double pref = -amplitude * k;
+    for (int a = 0; a < D; ++a) g[a] = pref * sin(k * x[a] + phase);
+}
+double orc_cosine_V(int N, int D, double amplitude, double wavelength, double phase, const double* coord) {
+    const double k = 2 * ORC_PI / wavelength;
+    double pot = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < D; ++a) pot += cos(k * coord[i * D + a] + phase);
+    pot *= amplitude;
+    return pot;
+}
+
 /* harmonic.cpp:3-27: k = m w^2, gradV = k x */
 void orc_harmonic_grad(int D, double mass, double omega, const double* x, double* g) {
     double k = mass * omega * omega;
@@ -392,6 +430,33 @@ double orc_effective_cutoff(const orc_params* p) {
     return c;
 }
 
+/* Simulation::initializePotential (simulation.cpp:612-645), external side */
+void orc_ext_grad(const orc_params* p, const double* x, double* g) {
+    const int D = p->ndim;
+    switch (p->ext_kind) {
+    case ORC_POT_HARMONIC: orc_harmonic_grad(D, p->mass, p->ext_omega, x, g); break;
+    case ORC_POT_DOUBLE_WELL: orc_double_well_grad(D, p->mass, p->ext_strength, p->ext_location, x, g); break;
+    case ORC_POT_COSINE: orc_cosine_grad(D, p->ext_amplitude, p->size, p->ext_phase, x, g); break;
+    default: for (int a = 0; a < D; ++a) g[a] = 0.0; break;
+    }
+}
+double orc_ext_V(const orc_params* p, const double* coord) {
+    const int N = p->natoms, D = p->ndim;
+    switch (p->ext_kind) {
+    case ORC_POT_HARMONIC: {
+        /* HarmonicPotential::V over the whole dVec */
+        double vv = 0.0;
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < D; ++a) vv += coord[i * D + a] * coord[i * D + a];
+        vv *= 0.5 * (p->mass * p->ext_omega * p->ext_omega);
+        return vv;
+    }
+    case ORC_POT_DOUBLE_WELL: return orc_double_well_V(N, D, p->mass, p->ext_strength, p->ext_location, coord);
+    case ORC_POT_COSINE: return orc_cosine_V(N, D, p->ext_amplitude, p->size, p->ext_phase, coord);
+    default: return 0.0;
+    }
+}
+
 /* simulation.cpp:499-512 (r1 - r2) */
 static void pair_sep(const orc_params* p, const double* coord, int i, int j, double* d) {
     for (int a = 0; a < p->ndim; ++a) {
@@ -406,8 +471,7 @@ void orc_physical_forces(const orc_params* p, const double* coord, double* f) {
     const int N = p->natoms, D = p->ndim;
     double g[8];
     for (int i = 0; i < N; ++i) {
-        if (p->ext_kind == ORC_POT_HARMONIC) orc_harmonic_grad(D, p->mass, p->ext_omega, coord + i * D, g);
-        else for (int a = 0; a < D; ++a) g[a] = 0.0;
+        orc_ext_grad(p, coord + i * D, g);
         for (int a = 0; a < D; ++a) f[i * D + a] = g[a] * (-1.0);
     }
     const double cutoff = orc_effective_cutoff(p);
@@ -926,15 +990,10 @@ int orc_sim_observables(orc_sim* s, double* out) {
         double pot = 0, virial = 0, int_pot = 0, ext_pot = 0;
         if (p->ext_kind != ORC_POT_FREE) {
             double g[8];
-            /* HarmonicPotential::V over the whole dVec */
-            double vv = 0.0;
-            for (int i = 0; i < N; ++i)
-                for (int a = 0; a < D; ++a) vv += coord[i * D + a] * coord[i * D + a];
-            vv *= 0.5 * (p->mass * p->ext_omega * p->ext_omega);
-            ext_pot = vv;
+            ext_pot = orc_ext_V(p, coord);
             pot += ext_pot;
             for (int i = 0; i < N; ++i) {
-                orc_harmonic_grad(D, p->mass, p->ext_omega, coord + i * D, g);
+                orc_ext_grad(p, coord + i * D, g);
                 for (int a = 0; a < D; ++a) virial -= coord[i * D + a] * (g[a] * (-1.0));
             }
         }

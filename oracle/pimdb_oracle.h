@@ -28,7 +28,8 @@ extern "C" {
 
 enum { ORC_OK = 0, ORC_E_NONFINITE = 1, ORC_E_ARG = 2 };
 
-enum { ORC_POT_FREE = 0, ORC_POT_HARMONIC = 1, ORC_POT_DIPOLE = 2, ORC_POT_AZIZ = 3 };
+enum { ORC_POT_FREE = 0, ORC_POT_HARMONIC = 1, ORC_POT_DIPOLE = 2, ORC_POT_AZIZ = 3,
+       ORC_POT_DOUBLE_WELL = 4, ORC_POT_COSINE = 5 };
 enum { ORC_PROP_CARTESIAN = 0, ORC_PROP_NORMAL_MODES = 1 };
 enum { ORC_THERMO_NONE = 0, ORC_THERMO_LANGEVIN = 1 };
 
@@ -41,12 +42,15 @@ typedef struct orc_params {
     double dt, gamma;
     int bosonic, pbc;
     double size;        /* box length L (bohr) */
-    int ext_kind;       /* ORC_POT_FREE or ORC_POT_HARMONIC */
+    int ext_kind;       /* ORC_POT_FREE, _HARMONIC, _DOUBLE_WELL or _COSINE */
     double ext_omega;
     int int_kind;       /* ORC_POT_* */
     double int_omega, int_strength, cutoff;
     int propagator, thermostat, nmthermostat, fixcom;
     unsigned int seed;
+    /* external double well (strength: energy, location: length) and cosine
+     * (amplitude: energy, phase: number; the wavelength is the box size) */
+    double ext_strength, ext_location, ext_amplitude, ext_phase;
 } orc_params;
 
 /* ---- exchange (src/bosonic_exchange/quadratic_bosonic_exchange.cpp) ---- */
@@ -79,6 +83,13 @@ double orc_vb_enumerate(int N, double beta_t, const double* E_kn);
 /* ---- potentials (src/potentials/{harmonic,dipole,aziz}.cpp) ---- */
 void orc_harmonic_grad(int D, double mass, double omega, const double* x, double* g);
 double orc_harmonic_V(int D, double mass, double omega, const double* x);
+void orc_double_well_grad(int D, double mass, double strength, double loc, const double* x, double* g);
+double orc_double_well_V(int N, int D, double mass, double strength, double loc, const double* coord);
+void orc_cosine_grad(int D, double amplitude, double wavelength, double phase, const double* x, double* g);
+double orc_cosine_V(int N, int D, double amplitude, double wavelength, double phase, const double* coord);
+/* External potential of one bead (N x D) and its gradient at one particle. */
+double orc_ext_V(const orc_params* p, const double* coord);
+void orc_ext_grad(const orc_params* p, const double* x, double* g);
 void orc_dipole_grad(int D, double strength, const double* x, double* g);
 double orc_dipole_V(int D, double strength, const double* x);
 void orc_aziz_grad(int D, const double* x, double* g);

@@ -21,10 +21,11 @@ namespace pimdb {
 
 struct RepDesc {
     int N, P, bosonic, pbc;
-    int thermostat, nm, fixcom, ext_harmonic;
+    int thermostat, nm, fixcom;
     double L, invL, mass, dt;
     double k, bt, c1, c2;
-    double ext_k, int_k, strength, cutoff;
+    double int_k, strength, cutoff;
+    ExtPot ext;
     uint32_t seed, rid;
     unsigned long long off_x;   // P*D*N block of x, p, f
     unsigned long long off_E;   // N(N+1)/2 block of E, ET
@@ -84,10 +85,12 @@ __device__ __forceinline__ T* uni(T* p) {
 __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
-    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.ext_harmonic = uni(r.ext_harmonic);
+    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
-    u.ext_k = uni(r.ext_k); u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
+    u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
+    u.ext.kind = uni(r.ext.kind); u.ext.a = uni(r.ext.a); u.ext.b = uni(r.ext.b); u.ext.c = uni(r.ext.c);
+    u.ext.d = uni(r.ext.d);
     u.seed = (uint32_t)uni((int)r.seed); u.rid = (uint32_t)uni((int)r.rid);
     u.off_x = (unsigned long long)uni((double*)r.off_x);
     u.off_E = (unsigned long long)uni((double*)r.off_E);
@@ -307,17 +310,23 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
         const bool exchange_bead = r.bosonic && (b == 0 || b == P - 1);
         const double* xprev = x + bs * ((b - 1 + P) % P);
         const double* xnext = x + bs * ((b + 1) % P);
-        for (int e = lane; e < D * N; e += WAVE) {
-            const double xi = xs[e];
-            const double ext = r.ext_harmonic ? (xi * r.ext_k) * (-1.0) : -0.0;
-            const double phys = ext + fo[e];
-            double spring = 0.0;
-            if (!exchange_bead) {
-                const double dp = disp(xi, xprev[e], pbc, r.L);
-                const double dn = disp(xi, xnext[e], pbc, r.L);
-                spring = r.k * (dp + dn);
+        for (int i = lane; i < N; i += WAVE) {
+            double xv[D], fe[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) xv[a] = xs[a * N + i];
+            ext_force<D>(r.ext, xv, fe);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const int e = a * N + i;
+                const double phys = fe[a] + fo[e];
+                double spring = 0.0;
+                if (!exchange_bead) {
+                    const double dp = disp(xv[a], xprev[e], pbc, r.L);
+                    const double dn = disp(xv[a], xnext[e], pbc, r.L);
+                    spring = r.k * (dp + dn);
+                }
+                fo[e] = exchange_bead ? phys : spring + phys;
             }
-            fo[e] = exchange_bead ? phys : spring + phys;
         }
     }
     __syncthreads();

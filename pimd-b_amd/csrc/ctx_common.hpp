@@ -28,7 +28,8 @@ inline const char* create_error() { return create_error_slot().c_str(); }
 // (langevin.cpp:10-13), the potential factory (:612-645) and the cutoff rule
 // (:83-90).  kB = hbar = 1 (include/units.h:18-22).
 struct Derived {
-    double beta, thermo_beta, omega_p, k, bt, c1, c2, cutoff, ext_k, int_k;
+    double beta, thermo_beta, omega_p, k, bt, c1, c2, cutoff, int_k;
+    ExtPot ext;
     bool bosonic;
 };
 
@@ -44,7 +45,20 @@ inline Derived derive(const pimdb_params& p) {
     double c = (p.int_kind == PIMDB_POT_FREE) ? 0.0 : p.cutoff;
     if (p.pbc) c = std::min(c, 0.5 * p.size);
     d.cutoff = c;
-    d.ext_k = p.mass * p.ext_omega * p.ext_omega;
+    d.ext = ExtPot{p.ext_kind, 0.0, 0.0, 0.0, 0.0};
+    if (p.ext_kind == PIMDB_POT_HARMONIC) {
+        d.ext.a = p.mass * p.ext_omega * p.ext_omega;
+    } else if (p.ext_kind == PIMDB_POT_DOUBLE_WELL) {
+        d.ext.a = 4 * p.mass * p.ext_strength;
+        d.ext.b = p.ext_location * p.ext_location;
+        d.ext.c = p.mass * p.ext_strength;
+    } else if (p.ext_kind == PIMDB_POT_COSINE) {
+        const double kc = 2 * 3.14159265358979323846 / p.size;
+        d.ext.a = -p.ext_amplitude * kc;
+        d.ext.b = kc;
+        d.ext.c = p.ext_phase;
+        d.ext.d = p.ext_amplitude;
+    }
     d.int_k = p.mass * p.int_omega * p.int_omega;
     d.bosonic = p.bosonic && p.nbeads > 1;   // simulation.cpp:690
     return d;
@@ -59,8 +73,10 @@ inline bool validate_params(const pimdb_params& p, std::string* why) {
     if (!(p.mass > 0.0)) return bad("The provided mass is unphysical");
     if (!(p.temperature > 0.0)) return bad("The specified temperature is unphysical");
     if (!(p.size > 0.0)) return bad("The provided system size is unphysical");
-    if (p.int_kind < 0 || p.int_kind > 3) return bad("unknown interaction potential");
-    if (p.ext_kind != PIMDB_POT_FREE && p.ext_kind != PIMDB_POT_HARMONIC) return bad("unsupported external potential");
+    if (p.int_kind < 0 || p.int_kind > 3) return bad("The specified interaction potential is not supported!");
+    if (p.ext_kind != PIMDB_POT_FREE && p.ext_kind != PIMDB_POT_HARMONIC && p.ext_kind != PIMDB_POT_DOUBLE_WELL &&
+        p.ext_kind != PIMDB_POT_COSINE)
+        return bad("The specified external potential is not supported!");
     if (p.propagator != PIMDB_PROP_CARTESIAN && p.propagator != PIMDB_PROP_NORMAL_MODES) return bad("unknown propagator");
     if (p.bosonic && p.propagator == PIMDB_PROP_NORMAL_MODES)
         return bad("Normal modes propogation is currently not available for bosons!");
@@ -142,7 +158,7 @@ inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     ForceArgs fa{};
     fa.N = p.natoms; fa.P = p.nbeads; fa.bosonic = d.bosonic ? 1 : 0; fa.pbc = p.pbc;
     fa.L = p.size; fa.k_spring = d.k;
-    fa.ext_k = d.ext_k; fa.ext_harmonic = p.ext_kind == PIMDB_POT_HARMONIC;
+    fa.ext = d.ext;
     fa.int_k = d.int_k; fa.strength = p.int_strength;
     fa.cutoff = d.cutoff;
     return fa;

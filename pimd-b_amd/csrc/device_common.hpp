@@ -15,7 +15,7 @@ namespace pimdb {
 
 constexpr int WAVE = 64;
 
-enum PotKind : int { POT_FREE = 0, POT_HARMONIC = 1, POT_DIPOLE = 2, POT_AZIZ = 3 };
+enum PotKind : int { POT_FREE = 0, POT_HARMONIC = 1, POT_DIPOLE = 2, POT_AZIZ = 3, POT_DOUBLE_WELL = 4, POT_COSINE = 5 };
 
 // Aziz HFDHE2 constants, src/potentials/aziz.cpp:6-13 (atomic units).
 struct Aziz {
@@ -152,6 +152,61 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
     } else {
         return 0.0;
     }
+}
+
+// External potential of one particle (Simulation::initializePotential,
+// simulation.cpp:612-645):
+//   harmonic     a = m w^2                          (harmonic.cpp:3-27)
+//   double well  a = 4 m s, b = loc^2, c = m s       (double_well.cpp:6-40)
+//   cosine       a = -A k, b = k = 2 pi / L, c = phase, d = A   (cosine.cpp:4-36)
+// The double well's energy sums (x_a^2 - loc^2)^2 over axes while its
+// gradient is the isotropic 4 m s (|x|^2 - loc^2) x, as in the reference.
+struct ExtPot {
+    int kind;
+    double a, b, c, d;
+};
+
+// Force -gradV_ext at one particle, computed as gradV * (-1) like
+// Simulation::updatePhysicalForces (simulation.cpp:430).
+template <int D>
+__device__ __forceinline__ void ext_force(const ExtPot& e, const double (&x)[D], double (&F)[D]) {
+    if (e.kind == POT_HARMONIC) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) F[a] = (x[a] * e.a) * (-1.0);
+    } else if (e.kind == POT_DOUBLE_WELL) {
+        double r2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) r2 += x[a] * x[a];
+        const double pref = e.a * (r2 - e.b);
+#pragma unroll
+        for (int a = 0; a < D; ++a) F[a] = (pref * x[a]) * (-1.0);
+    } else if (e.kind == POT_COSINE) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) F[a] = (e.a * sin(e.b * x[a] + e.c)) * (-1.0);
+    } else {
+#pragma unroll
+        for (int a = 0; a < D; ++a) F[a] = -0.0;
+    }
+}
+
+// The particle's share of V_ext of its bead.
+template <int D>
+__device__ __forceinline__ double ext_energy(const ExtPot& e, const double (&x)[D]) {
+    double s = 0.0;
+    if (e.kind == POT_HARMONIC) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) s += x[a] * x[a];
+        return 0.5 * e.a * s;
+    } else if (e.kind == POT_DOUBLE_WELL) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) s += (x[a] * x[a] - e.b) * (x[a] * x[a] - e.b);
+        return s * e.c;
+    } else if (e.kind == POT_COSINE) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) s += cos(e.b * x[a] + e.c);
+        return s * e.d;
+    }
+    return 0.0;
 }
 
 // Pair potential energy V(d) (harmonic.cpp:9-19, dipole.cpp:5-22, aziz.cpp:18-54).

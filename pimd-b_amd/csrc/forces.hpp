@@ -21,8 +21,7 @@ namespace pimdb {
 struct ForceArgs {
     int N, P, bosonic, pbc;
     double L, k_spring;
-    double ext_k;            // trap constant m w^2 (0 for a free external potential)
-    int ext_harmonic;
+    ExtPot ext;              // external potential
     double int_k, strength;  // pair harmonic constant, dipole strength
     double cutoff;           // effective cutoff (0: no pair loop; < 0: all pairs)
 };
@@ -87,13 +86,15 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
     const bool exchange_bead = fa.bosonic && (b == 0 || b == P - 1);
     const double* xprev = x + bs * ((b - 1 + P) % P);
     const double* xnext = x + bs * ((b + 1) % P);
+    double xv[D], fe[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) xv[a] = sx[a * N + i];
+    ext_force<D>(fa.ext, xv, fe);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const size_t idx = bs * b + (size_t)a * N + i;
-        const double xi = sx[a * N + i];
-        // (-1) * gradV_ext(x): harmonic.cpp:21-23
-        const double ext = fa.ext_harmonic ? (xi * fa.ext_k) * (-1.0) : -0.0;
-        const double phys = ext + acc[a];
+        const double xi = xv[a];
+        const double phys = fe[a] + acc[a];
         if (fext) fext[idx] = phys;
         double spring = 0.0;
         if (!exchange_bead) {
@@ -128,13 +129,12 @@ k_bead_energies(ForceArgs fa, const double* __restrict__ x, double* __restrict__
         double xi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) xi[a] = sx[a * N + i];
-        if (wave == 0 && fa.ext_harmonic) {
-            double s = 0.0;
+        if (wave == 0 && fa.ext.kind != POT_FREE) {
+            double fe[D];
+            ext = ext_energy<D>(fa.ext, xi);
+            ext_force<D>(fa.ext, xi, fe);
 #pragma unroll
-            for (int a = 0; a < D; ++a) s += xi[a] * xi[a];
-            ext = 0.5 * fa.ext_k * s;
-#pragma unroll
-            for (int a = 0; a < D; ++a) vir -= xi[a] * ((xi[a] * fa.ext_k) * (-1.0));
+            for (int a = 0; a < D; ++a) vir -= xi[a] * fe[a];
         }
         if constexpr (KIND != POT_FREE) {
             if (fa.cutoff != 0.0) {

@@ -204,10 +204,9 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         RepDesc d{};
         d.N = p.natoms; d.P = p.nbeads; d.bosonic = dv.bosonic; d.pbc = p.pbc;
         d.thermostat = p.thermostat; d.nm = p.nmthermostat; d.fixcom = p.fixcom;
-        d.ext_harmonic = p.ext_kind == PIMDB_POT_HARMONIC;
         d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
         d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
-        d.ext_k = dv.ext_k; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
+        d.ext = dv.ext; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
         d.seed = p.seed; d.rid = 0;   // noise stream keyed by the seed, as in the single-system form
         d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
         d.off_E = b->nE; b->nE += (size_t)p.natoms * (p.natoms + 1) / 2;

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_PKG, "libpimdb_gpu.so")
 _DP = C.POINTER(C.c_double)
 
 OK, E_NONFINITE, E_ARG, E_HIP, E_STATE = 0, 1, 2, 3, 4
-POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3}
+POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
 THERMO = {"none": 0, "langevin": 1}
 
@@ -41,6 +41,8 @@ class Params(C.Structure):
         ("propagator", C.c_int), ("thermostat", C.c_int),
         ("nmthermostat", C.c_int), ("fixcom", C.c_int),
         ("seed", C.c_uint),
+        ("ext_strength", C.c_double), ("ext_location", C.c_double),
+        ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
     ]
 
 
@@ -195,12 +197,12 @@ class Context:
         self._chk(lib().pimdb_observables(self.h, _p(out)))
         return out
 
-
     def exchange_estimators(self):
         """(primEstimator, distinct probability, longest probability)."""
         v = np.zeros(3)
         self._chk(lib().pimdb_exchange_estimators(self.h, _p(v[0:]), _p(v[1:]), _p(v[2:])))
         return float(v[0]), float(v[1]), float(v[2])
+
     def graph_capture(self):
         self._chk(lib().pimdb_graph_capture(self.h))
 

@@ -45,9 +45,28 @@ def aziz_replicas(n):
     return out
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz"])
+def trap_replicas(n):
+    """Replicas with the reference's other external potentials (double well,
+    cosine) under harmonic pairs, uneven N and P."""
+    rng = np.random.default_rng(21)
+    out = []
+    for r in range(n):
+        N, P = int([12, 40, 24][r % 3]), int([8, 4, 16][r % 3])
+        kw = dict(natoms=N, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+                  gamma=1.0 / 2000.0, bosonic=int(r % 2 == 0), pbc=0, size=30.0,
+                  ext_kind=["double_well", "cosine"][r % 2], ext_strength=1.0 * MEV, ext_location=1.9,
+                  ext_amplitude=2.0 * MEV, ext_phase=0.3, int_kind="harmonic", int_omega=1.0 * MEV,
+                  cutoff=-1.0, propagator="cartesian", thermostat="langevin", seed=500 + r)
+        x = rng.uniform(-6.0, 6.0, size=(P, N, 3))
+        p = rng.normal(0, 0.5, size=(P, N, 3))
+        out.append((kw, x, p))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz", "traps"])
 def test_batch_forces_match_oracle(which):
-    reps = dipole_replicas(12, seed=3) if which == "dipole" else aziz_replicas(6)
+    reps = {"dipole": lambda: dipole_replicas(12, seed=3), "aziz": lambda: aziz_replicas(6),
+            "traps": lambda: trap_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)

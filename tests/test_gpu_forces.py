@@ -30,7 +30,16 @@ CASES = {
     "free_trap": dict(N=33, P=7, D=3, int_kind="free", pbc=0, L=40.0, cutoff=-1.0, bosonic=1, ext="harmonic"),
     "one_bead": dict(N=25, P=1, D=3, int_kind="aziz", pbc=1, L=20.0, cutoff=100.0, bosonic=1, ext="free"),
     "two_beads": dict(N=25, P=2, D=3, int_kind="dipole", pbc=1, L=20.0, cutoff=100.0, bosonic=1, ext="free"),
+    "double_well_harmonic": dict(N=30, P=8, D=3, int_kind="harmonic", pbc=0, L=30.0, cutoff=-1.0, bosonic=1,
+                                 ext="double_well"),
+    "double_well_2d_dist": dict(N=24, P=6, D=2, int_kind="free", pbc=0, L=30.0, cutoff=-1.0, bosonic=0,
+                                ext="double_well"),
+    "cosine_dipole": dict(N=40, P=8, D=3, int_kind="dipole", pbc=1, L=30.0, cutoff=100.0, bosonic=1, ext="cosine"),
+    "cosine_1d": dict(N=16, P=4, D=1, int_kind="harmonic", pbc=1, L=12.0, cutoff=100.0, bosonic=1, ext="cosine"),
 }
+
+# external double well: strength 1 meV at 1 angstrom; cosine: amplitude 2 meV, phase 0.3
+EXT = dict(ext_strength=1.0 * MEV, ext_location=1.8897261, ext_amplitude=2.0 * MEV, ext_phase=0.3)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -42,7 +51,7 @@ def test_forces_match_oracle(name):
     kw = dict(natoms=c["N"], nbeads=c["P"], ndim=c["D"], mass=4.0026 * AMU, temperature=2.0 * KELVIN,
               dt=20.0, gamma=1e-3, bosonic=c["bosonic"], pbc=c["pbc"], size=L,
               ext_kind=c["ext"], ext_omega=3.0 * MEV, int_kind=c["int_kind"], int_omega=1.0 * MEV,
-              int_strength=0.8, cutoff=c["cutoff"], propagator="cartesian", thermostat="langevin", seed=3)
+              int_strength=0.8, cutoff=c["cutoff"], propagator="cartesian", thermostat="langevin", seed=3, **EXT)
     f = gpu_forces(kw, x)
     fo = O.update_forces(O.make_params(**kw), x)
     assert per_bead_err(f, fo) < 1e-10

@@ -37,6 +37,21 @@ def test_observables_aziz(replicate):
     compare(kw, x, p)
 
 
+@pytest.mark.parametrize("ext", ["double_well", "cosine"])
+def test_observables_external_traps(ext):
+    """Energy, virial and their ext/int split with the reference's double-well
+    and cosine external potentials (harmonic pairs)."""
+    rng = np.random.default_rng(6)
+    N, P = 20, 8
+    x = rng.uniform(-5.0, 5.0, size=(P, N, 3))
+    p = rng.normal(0, 0.4, size=x.shape)
+    kw = dict(natoms=N, nbeads=P, ndim=3, mass=7296.3, temperature=2.0 * KELVIN, dt=20.0, gamma=5e-4,
+              bosonic=1, pbc=0, size=25.0, ext_kind=ext, ext_strength=3.7e-5, ext_location=1.9,
+              ext_amplitude=7.3e-5, ext_phase=0.3, int_kind="harmonic", int_omega=3.7e-5, cutoff=-1.0,
+              propagator="cartesian", thermostat="langevin", seed=9)
+    compare(kw, x, p)
+
+
 @pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic", "dist_harmonic",
                                   "bosonic_quadratic_harmonic_nmthermostat_dynamics"])
 def test_observables_reference_rows(case):

@@ -249,3 +249,93 @@ def test_nm_propagator_free_ring_is_exact():
             qt[k] = math.cos(w * t) * q0[k] + math.sin(w * t) / (5.0 * w) * v0[k]
     xt = np.einsum("kj,knd->jnd", Cm, qt)
     assert np.max(np.abs(x - xt)) < 1e-10
+
+
+# ---- external double well and cosine (double_well.cpp, cosine.cpp) ----
+
+def _dw(x, m=2.0, s=0.7, loc=1.3):
+    L = O.lib()
+    g = np.zeros(len(x))
+    L.orc_double_well_grad(len(x), m, s, loc, O.ptr(x), O.ptr(g))
+    return g
+
+
+def test_double_well_gradient_is_isotropic_well():
+    """The reference's gradV is the gradient of the isotropic well
+    U(r) = m s (r^2 - loc^2)^2 (its V sums per axis instead; the force is what
+    enters the dynamics).  Central differences of U at random 3-D points."""
+    m, s, loc = 2.0, 0.7, 1.3
+    U = lambda y: m * s * (float(y @ y) - loc * loc) ** 2
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        x = rng.normal(0, 1.2, size=3)
+        g = _dw(x, m, s, loc)
+        h = 1e-6
+        for a in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a] += h
+            xm[a] -= h
+            assert abs((U(xp) - U(xm)) / (2 * h) - g[a]) <= 1e-7 * np.max(np.abs(g)) + 1e-12
+
+
+def test_double_well_energy_closed_forms():
+    """Per-axis energy: V(0) = N D m s loc^4; on one axis V and its derivative
+    agree with the gradient (the isotropic and per-axis forms coincide there);
+    minima at +-loc along an axis with the other axes at 0 cost m s loc^4 each."""
+    L = O.lib()
+    m, s, loc = 2.0, 0.7, 1.3
+    N, D = 3, 3
+    zero = np.zeros(N * D)
+    assert abs(L.orc_double_well_V(N, D, m, s, loc, O.ptr(zero)) - N * D * m * s * loc ** 4) < 1e-13
+    x1 = np.array([loc, 0.0, 0.0])
+    assert abs(L.orc_double_well_V(1, 3, m, s, loc, O.ptr(x1)) - 2 * m * s * loc ** 4) < 1e-13
+    for a0 in (0.4, -1.7, 2.2):
+        h = 1e-6
+        vp = L.orc_double_well_V(1, 1, m, s, loc, O.ptr(np.array([a0 + h])))
+        vm = L.orc_double_well_V(1, 1, m, s, loc, O.ptr(np.array([a0 - h])))
+        g = _dw(np.array([a0]), m, s, loc)[0]
+        assert abs((vp - vm) / (2 * h) - g) <= 1e-7 * abs(g) + 1e-12
+
+
+def test_cosine_potential():
+    """V = A sum cos(k x + phase), k = 2 pi / L: gradient is the derivative of
+    V (central differences), V = A D at k x + phase = 0 on every axis, and V is
+    periodic in the box."""
+    L = O.lib()
+    A, Lbox, ph = 0.02, 7.5, 0.3
+    k = 2 * np.pi / Lbox
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-Lbox, Lbox, size=3)
+    g = np.zeros(3)
+    L.orc_cosine_grad(3, A, Lbox, ph, O.ptr(x), O.ptr(g))
+    h = 1e-6
+    for a in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[a] += h
+        xm[a] -= h
+        fd = (L.orc_cosine_V(1, 3, A, Lbox, ph, O.ptr(xp)) - L.orc_cosine_V(1, 3, A, Lbox, ph, O.ptr(xm))) / (2 * h)
+        assert abs(fd - g[a]) <= 1e-7 * np.max(np.abs(g)) + 1e-13
+    x0 = np.full(3, -ph / k)
+    assert abs(L.orc_cosine_V(1, 3, A, Lbox, ph, O.ptr(x0)) - 3 * A) < 1e-15
+    xs = x + Lbox
+    assert abs(L.orc_cosine_V(1, 3, A, Lbox, ph, O.ptr(xs)) - L.orc_cosine_V(1, 3, A, Lbox, ph, O.ptr(x))) < 1e-14
+
+
+def test_physical_forces_use_external_gradient():
+    """orc_physical_forces with a double-well or cosine trap and free pairs is
+    -gradV_ext of every particle (simulation.cpp:430)."""
+    rng = np.random.default_rng(12)
+    x = rng.normal(0, 1.0, size=(6, 3))
+    for ext in ("double_well", "cosine"):
+        kw = dict(natoms=6, nbeads=1, ndim=3, mass=2.0, temperature=1e-5, dt=1.0, gamma=0.0, bosonic=0,
+                  pbc=0, size=7.5, ext_kind=ext, int_kind="free", ext_strength=0.7, ext_location=1.3,
+                  ext_amplitude=0.02, ext_phase=0.3)
+        f = np.zeros((6, 3))
+        O.lib().orc_physical_forces(O.make_params(**kw), O.ptr(x), O.ptr(f))
+        for i in range(6):
+            g = np.zeros(3)
+            if ext == "double_well":
+                O.lib().orc_double_well_grad(3, 2.0, 0.7, 1.3, O.ptr(x[i].copy()), O.ptr(g))
+            else:
+                O.lib().orc_cosine_grad(3, 0.02, 7.5, 0.3, O.ptr(x[i].copy()), O.ptr(g))
+            assert np.array_equal(f[i], -g)
