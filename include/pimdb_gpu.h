@@ -34,7 +34,7 @@ enum pimdb_status {
     PIMDB_E_STATE = 4      /* call not valid in the current state             */
 };
 
-/* Potentials (src/potentials/*.cpp).  DOUBLE_WELL and COSINE are external
+/* Potentials (the classes of src/potentials).  DOUBLE_WELL and COSINE are external
  * potentials only, as in the reference's allowed names (params.cpp:189-190). */
 enum pimdb_potential {
     PIMDB_POT_FREE = 0, PIMDB_POT_HARMONIC = 1, PIMDB_POT_DIPOLE = 2, PIMDB_POT_AZIZ = 3,
