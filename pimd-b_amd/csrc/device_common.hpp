@@ -70,6 +70,37 @@ __device__ __forceinline__ double norm2(const double (&d)[D]) {
     return s;
 }
 
+// e^x for x <= 709 (the pair potentials evaluate it at x <= 0), as 2^k e^r
+// with k = rint(x / ln 2) and r = x - k ln 2 reduced with the two-part ln 2 of
+// Cody and Waite (1980), |r| <= ln(2)/2; e^r is its Taylor series through
+// r^13/13! (the first omitted term is below 5e-18 relative).  Relative error
+// within about 2 ulp of the correctly rounded value; arguments below -746
+// give 0, NaN stays NaN.
+__device__ __forceinline__ double exp_taylor(double x) {
+    const double LOG2E = 1.4426950408889634;
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    if (x < -746.0) x = -746.0;
+    const double k = rint(x * LOG2E);
+    double r = fma(-k, LN2_HI, x);
+    r = fma(-k, LN2_LO, r);
+    double p = 1.0 / 6227020800.0;
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
 // Scalar factor g(r) of the interaction gradient, gradV(d) = g * d
 // (harmonic.cpp:21-23, dipole.cpp:24-45, aziz.cpp:56-106).  `norm` is |d|.
 template <int KIND>
@@ -113,7 +144,8 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
 }
 
 // g(r) as above from |d| and 1/|d| with products in place of the
-// reference's divisions (norm/rm, 1/r, g/norm): at most a few ulp apart.
+// reference's divisions (norm/rm, 1/r, g/norm) and exp_taylor in place of
+// the library exp: at most a few ulp apart.
 template <int KIND>
 __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_norm, double kpair, double strength) {
     if constexpr (KIND == POT_HARMONIC) {
@@ -124,7 +156,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
     } else if constexpr (KIND == POT_AZIZ) {
         constexpr double inv_rm = 1.0 / Aziz::rm;
         const double r = norm * inv_rm;
-        const double T1 = -Aziz::A * Aziz::alpha * exp(-Aziz::alpha * r);
+        const double T1 = -Aziz::A * Aziz::alpha * exp_taylor(-Aziz::alpha * r);
         double g;
         if (r > HARD_CORE_EPS && r < 0.01) {
             g = T1 * (Aziz::epsilon / Aziz::rm);
@@ -140,7 +172,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
             double F = 1.0, dF = 0.0;
             if (r < Aziz::D) {
                 const double t = Aziz::D * ix - 1.0;
-                const double ex = exp(-t * t);
+                const double ex = exp_taylor(-t * t);
                 F = ex;
                 dF = 2.0 * Aziz::D * ix * ix * t * ex;
             }

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "pairs.hpp"
 
 namespace pimdb {
 
@@ -55,18 +56,15 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
             for (int a = 0; a < D; ++a) xi[a] = sx[a * N + i];
             const int chunk = (N + FORCE_WAVES - 1) / FORCE_WAVES;
             const int jlo = wave * chunk, jhi = min(N, jlo + chunk);
-            const double invL = 1.0 / fa.L;
+            const PairParams pp = make_pair_params(fa.L, fa.pbc, fa.cutoff, fa.int_k, fa.strength);
             for (int j = jlo; j < jhi; ++j) {
                 if (j == i) continue;
-                double d[D];
+                double xj[D], fij[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) d[a] = disp_fast(sx[a * N + j], xi[a], pbc, fa.L, invL);   // r_i - r_j
-                const double r2 = norm2<D>(d);
-                const double norm = sqrt(r2);
-                if (norm < fa.cutoff || fa.cutoff < 0.0) {
-                    const double g = pair_grad_factor_fast<KIND>(norm, 1.0 / norm, fa.int_k, fa.strength);
+                for (int a = 0; a < D; ++a) xj[a] = sx[a * N + j];
+                if (pair_force<D, KIND>(xi, xj, pp, fij)) {
 #pragma unroll
-                    for (int a = 0; a < D; ++a) acc[a] += -(g * d[a]);
+                    for (int a = 0; a < D; ++a) acc[a] += fij[a];
                 }
             }
         }

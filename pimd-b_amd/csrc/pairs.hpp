@@ -64,6 +64,12 @@ __device__ __forceinline__ double rsqrt_refined(double r2) {
     return y;
 }
 
+// The reference's strict test |d| < cutoff (simulation.cpp:444) for the rare
+// r^2 within 1e-12 of cutoff^2, where r^2 alone cannot decide; |d| is sqrt of
+// the same sum of squares the reference takes the norm of.  Out of line so
+// the compiler does not evaluate the square root for every pair.
+static __device__ __noinline__ bool within_cutoff_exact(double r2, double cutoff) { return sqrt(r2) < cutoff; }
+
 // Force on i due to j, -gradV(r_i - r_j), if the pair is inside the cutoff
 // (distance < cutoff, or cutoff < 0); false otherwise.
 template <int D, int KIND>
@@ -74,7 +80,7 @@ __device__ __forceinline__ bool pair_force(const double (&xi)[D], const double (
     for (int a = 0; a < D; ++a) d[a] = min_image_pair(xi[a] - xj[a], pp);
     const double r2 = norm2<D>(d);
     bool in = pp.cutoff < 0.0 || r2 < pp.cut2_lo;
-    if (!in && pp.cutoff > 0.0 && r2 < pp.cut2_hi) in = sqrt(r2) < pp.cutoff;   // exact test inside the band
+    if (!in && r2 < pp.cut2_hi) in = within_cutoff_exact(r2, pp.cutoff);
     if (!in) return false;
     const double inv = rsqrt_refined(r2);
     const double g = pair_grad_factor_fast<KIND>(r2 * inv, inv, pp.int_k, pp.strength);
