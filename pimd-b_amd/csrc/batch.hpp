@@ -300,7 +300,11 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
         const double* xs = x + bs * b;
         double* fo = f + bs * b;
         if (KIND != POT_FREE && r.cutoff != 0.0) {
-            slice_pair_forces<D, KIND>(xs, fo, N, pp, lane);
+            switch (pair_image(pp)) {
+            case 0: slice_pair_forces<D, KIND, 0>(xs, fo, N, pp, lane); break;
+            case 1: slice_pair_forces<D, KIND, 1>(xs, fo, N, pp, lane); break;
+            default: slice_pair_forces<D, KIND, 2>(xs, fo, N, pp, lane); break;
+            }
         } else {
             for (int e = lane; e < D * N; e += WAVE) fo[e] = 0.0;
             __threadfence_block();

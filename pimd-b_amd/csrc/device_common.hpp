@@ -76,6 +76,15 @@ __device__ __forceinline__ double norm2(const double (&d)[D]) {
 // r^13/13! (the first omitted term is below 5e-18 relative).  Relative error
 // within about 2 ulp of the correctly rounded value; arguments below -746
 // give 0, NaN stays NaN.
+// a * b + c with c in a scalar register.  The compiler turns p = fma(p, r, c)
+// with a loop-invariant c into a copy of c plus v_fmac; a three-operand
+// v_fma_f64 with the constant in an SGPR needs neither.
+__device__ __forceinline__ double fma_sconst(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
 __device__ __forceinline__ double exp_taylor(double x) {
     const double LOG2E = 1.4426950408889634;
     const double LN2_HI = 6.93147180369123816490e-01;
@@ -84,17 +93,16 @@ __device__ __forceinline__ double exp_taylor(double x) {
     const double k = rint(x * LOG2E);
     double r = fma(-k, LN2_HI, x);
     r = fma(-k, LN2_LO, r);
-    double p = 1.0 / 6227020800.0;
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
+    double p = fma_sconst(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
+    p = fma_sconst(p, r, 1.0 / 39916800.0);
+    p = fma_sconst(p, r, 1.0 / 3628800.0);
+    p = fma_sconst(p, r, 1.0 / 362880.0);
+    p = fma_sconst(p, r, 1.0 / 40320.0);
+    p = fma_sconst(p, r, 1.0 / 5040.0);
+    p = fma_sconst(p, r, 1.0 / 720.0);
+    p = fma_sconst(p, r, 1.0 / 120.0);
+    p = fma_sconst(p, r, 1.0 / 24.0);
+    p = fma_sconst(p, r, 1.0 / 6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
@@ -144,8 +152,9 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
 }
 
 // g(r) as above from |d| and 1/|d| with products in place of the
-// reference's divisions (norm/rm, 1/r, g/norm) and exp_taylor in place of
-// the library exp: at most a few ulp apart.
+// reference's divisions (norm/rm, 1/r, g/norm), the dispersion terms
+// factored as polynomials in ix^2, and exp_taylor in place of the library
+// exp: a few ulp apart.
 template <int KIND>
 __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_norm, double kpair, double strength) {
     if constexpr (KIND == POT_HARMONIC) {
@@ -161,24 +170,21 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
         if (r > HARD_CORE_EPS && r < 0.01) {
             g = T1 * (Aziz::epsilon / Aziz::rm);
         } else {
+            // T2 + T3 = ix^6 [ix (6C6 + 8C8 ix^2 + 10C10 ix^4) F
+            //                 - (C6 + C8 ix^2 + C10 ix^4) dF], dF = 2 D ix^2 t F
             const double ix = Aziz::rm * inv_norm;
             const double ix2 = ix * ix;
             const double ix6 = ix2 * ix2 * ix2;
-            const double ix7 = ix6 * ix;
-            const double ix8 = ix6 * ix2;
-            const double ix9 = ix8 * ix;
-            const double ix10 = ix8 * ix2;
-            const double ix11 = ix10 * ix;
-            double F = 1.0, dF = 0.0;
+            const double pa = fma_sconst(fma_sconst(10.0 * Aziz::C10, ix2, 8.0 * Aziz::C8), ix2, 6.0 * Aziz::C6);
+            double attr = ix * pa;
             if (r < Aziz::D) {
+                const double pb = fma_sconst(fma_sconst(Aziz::C10, ix2, Aziz::C8), ix2, Aziz::C6);
                 const double t = Aziz::D * ix - 1.0;
                 const double ex = exp_taylor(-t * t);
-                F = ex;
-                dF = 2.0 * Aziz::D * ix * ix * t * ex;
+                attr = (attr - (2.0 * Aziz::D) * ix2 * t * pb) * ex;
             }
-            const double T2 = (6.0 * Aziz::C6 * ix7 + 8.0 * Aziz::C8 * ix9 + 10.0 * Aziz::C10 * ix11) * F;
-            const double T3 = -(Aziz::C6 * ix6 + Aziz::C8 * ix8 + Aziz::C10 * ix10) * dF;
-            g = (Aziz::epsilon / Aziz::rm) * (T1 + T2 + T3);
+            const double T2T3 = ix6 * attr;
+            g = (Aziz::epsilon / Aziz::rm) * (T1 + T2T3);
         }
         return g * inv_norm;
     } else {

@@ -57,15 +57,23 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
             const int chunk = (N + FORCE_WAVES - 1) / FORCE_WAVES;
             const int jlo = wave * chunk, jhi = min(N, jlo + chunk);
             const PairParams pp = make_pair_params(fa.L, fa.pbc, fa.cutoff, fa.int_k, fa.strength);
-            for (int j = jlo; j < jhi; ++j) {
-                if (j == i) continue;
-                double xj[D], fij[D];
+            auto run = [&](auto img) {
+                constexpr int IMG = decltype(img)::value;
+                for (int j = jlo; j < jhi; ++j) {
+                    if (j == i) continue;
+                    double xj[D], fij[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) xj[a] = sx[a * N + j];
-                if (pair_force<D, KIND>(xi, xj, pp, fij)) {
+                    for (int a = 0; a < D; ++a) xj[a] = sx[a * N + j];
+                    if (pair_force<D, KIND, IMG>(xi, xj, pp, fij)) {
 #pragma unroll
-                    for (int a = 0; a < D; ++a) acc[a] += fij[a];
+                        for (int a = 0; a < D; ++a) acc[a] += fij[a];
+                    }
                 }
+            };
+            switch (pair_image(pp)) {
+            case 0: run(std::integral_constant<int, 0>()); break;
+            case 1: run(std::integral_constant<int, 1>()); break;
+            default: run(std::integral_constant<int, 2>()); break;
             }
         }
         if (wave > 0) {

@@ -16,6 +16,8 @@
 // does not matter: the column index travels with the data.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace pimdb {
@@ -46,13 +48,18 @@ __host__ __device__ inline PairParams make_pair_params(double L, int pbc, double
 
 // Minimum image.  With a cutoff <= L/2 a displacement whose image choice is
 // ambiguous (|dx| within rounding of L/2) lies outside the cutoff whichever
-// image is taken, so the guarded exact path is only needed without a cutoff.
+// image is taken, so there dx - L rint(dx/L) (one rounding less than the
+// reference's dx - L floor(dx/L + 1/2)) serves; without a cutoff the guarded
+// exact path keeps the reference's image.
+// IMG: 0 no PBC, 1 PBC with a cutoff, 2 PBC without a cutoff (pair_image()).
+template <int IMG>
 __device__ __forceinline__ double min_image_pair(double dx, const PairParams& pp) {
-#pragma clang fp contract(off)
-    if (!pp.pbc) return dx;
-    if (pp.guard) return min_image_fast(dx, pp.L, pp.invL);
-    return dx - pp.L * floor(dx * pp.invL + 0.5);
+    if constexpr (IMG == 0) return dx;
+    else if constexpr (IMG == 2) return min_image_fast(dx, pp.L, pp.invL);
+    else return fma(-pp.L, rint(dx * pp.invL), dx);
 }
+
+__host__ __device__ inline int pair_image(const PairParams& pp) { return !pp.pbc ? 0 : (pp.guard ? 2 : 1); }
 
 // 1/sqrt(r2) to double precision: hardware estimate + two Newton steps
 // (each squares the relative error).
@@ -72,12 +79,12 @@ static __device__ __noinline__ bool within_cutoff_exact(double r2, double cutoff
 
 // Force on i due to j, -gradV(r_i - r_j), if the pair is inside the cutoff
 // (distance < cutoff, or cutoff < 0); false otherwise.
-template <int D, int KIND>
+template <int D, int KIND, int IMG>
 __device__ __forceinline__ bool pair_force(const double (&xi)[D], const double (&xj)[D], const PairParams& pp,
                                            double (&f)[D]) {
     double d[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) d[a] = min_image_pair(xi[a] - xj[a], pp);
+    for (int a = 0; a < D; ++a) d[a] = min_image_pair<IMG>(xi[a] - xj[a], pp);
     const double r2 = norm2<D>(d);
     bool in = pp.cutoff < 0.0 || r2 < pp.cut2_lo;
     if (!in && r2 < pp.cut2_hi) in = within_cutoff_exact(r2, pp.cutoff);
@@ -101,7 +108,7 @@ __device__ __forceinline__ double lane_rotate(double v) {
 // All pairs of one slice on one wave.  xs: slice positions [D][N]; fo: the
 // slice's force array [D][N], owned by this wave, overwritten with the pair
 // forces.  Must be called by all 64 lanes together.
-template <int D, int KIND>
+template <int D, int KIND, int IMG>
 __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, int N, const PairParams& pp, int lane) {
     for (int e = lane; e < D * N; e += WAVE) fo[e] = 0.0;
     __threadfence_block();
@@ -128,7 +135,7 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
             for (int s = s0; s < s1; ++s) {
                 if (row_ok && jc < N && (!diag || s < 32 || lane < 32)) {
                     double f[D];
-                    if (pair_force<D, KIND>(xi, xj, pp, f)) {
+                    if (pair_force<D, KIND, IMG>(xi, xj, pp, f)) {
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             racc[a] += f[a];
