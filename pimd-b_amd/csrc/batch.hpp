@@ -28,13 +28,12 @@ struct RepDesc {
     ExtPot ext;
     uint32_t seed, rid;
     unsigned long long off_x;   // P*D*N block of x, p, f
-    unsigned long long off_E;   // N(N+1)/2 block of E, ET
     unsigned long long off_V;   // N+1 block of V, Vb
     unsigned long long off_C;   // P*P block of Ct, CTt
 };
 
 struct BatchPtrs {
-    double *x, *p, *f, *E, *ET, *V, *Vb;
+    double *x, *p, *f, *V, *Vb;
     const double *Ct, *CTt;
     const RepDesc* reps;
     const int* sched;          // replicas of workgroup g: sched[sched_off[g] .. sched_off[g+1])
@@ -93,7 +92,6 @@ __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     u.ext.d = uni(r.ext.d);
     u.seed = (uint32_t)uni((int)r.seed); u.rid = (uint32_t)uni((int)r.rid);
     u.off_x = (unsigned long long)uni((double*)r.off_x);
-    u.off_E = (unsigned long long)uni((double*)r.off_E);
     u.off_V = (unsigned long long)uni((double*)r.off_V);
     u.off_C = (unsigned long long)uni((double*)r.off_C);
     return u;
@@ -345,15 +343,12 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P) { return 2 * (size
 // on entry.
 template <int D>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
-                                         double* f_, double* E_, double* ET_, double* V_, double* Vb_, int* status_,
-                                         double* lds_) {
+                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_) {
     const int N = uni(N_), P = uni(P_);
     const bool pbc = uni(pbc_) != 0;
     const double L = uni(L_), k = uni(k_), bt = uni(bt_);
     const double* x = uni(x_);
     double* f = uni(f_);
-    double* E = uni(E_);
-    double* ET = uni(ET_);
     double* V = uni(V_);
     double* Vb = uni(Vb_);
     int* status = uni(status_);
@@ -361,20 +356,19 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     const size_t bs = (size_t)D * N;
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
     const int wave = uni((int)(threadIdx.x / WAVE));
-    const double* x1 = x;
-    const double* xP = x + bs * (P - 1);
-    for (int v = wave; v < N; v += nw) cycle_energy_column<D>(v, N, lane, x1, xP, k, pbc, L, E, ET);
-    __threadfence_block();
-    __syncthreads();
-    exchange_recursion<0, BATCH_REC_UNROLL>(N, bt, E, ET, V, Vb, status, rec_carve(lds, N));
-    exchange_recursion<1, BATCH_REC_UNROLL>(N, bt, E, ET, V, Vb, status, rec_carve(lds, N));
+    // beads 1 and P and the link prefix sums in LDS; cycle energies are
+    // recomputed from them wherever they are needed
+    const RecShared sh = rec_carve(lds, N, D);
+    const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
+    exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh);
+    exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh);
     __threadfence_block();
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
     for (int task = wave; task < 2 * N; task += nw) {
         const int which = task / N, l = task % N;
         double F[D];
-        exterior_force<D, BATCH_EXT_UNROLL>(which, l, N, lane, bt, k, pbc, L, E, ET, V, Vb, x1, x2, xPm1, xP, false, F);
+        exterior_force<D, BATCH_EXT_UNROLL>(which, l, N, lane, bt, k, r, V, Vb, x2, xPm1, false, F);
         if (lane == 0) {
             const size_t b = which == 0 ? 0 : (size_t)(P - 1);
 #pragma unroll
@@ -411,8 +405,8 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         }
         if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
-            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.E + r.off_E,
-                           bp.ET + r.off_E, bp.V + r.off_V, bp.Vb + r.off_V, bp.status + ri, lds);
+            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                           bp.Vb + r.off_V, bp.status + ri, lds);
         if constexpr (integ) {
             double cm[D];
             wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm);

@@ -164,7 +164,7 @@ inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     return fa;
 }
 
-inline size_t rec_lds_bytes(int N) { return sizeof(double) * rec_shared_doubles(N); }
+inline size_t rec_lds_bytes(int N, int D) { return sizeof(double) * rec_shared_doubles(N, D); }
 
 inline size_t device_lds_limit() {
     static size_t lim = 0;

@@ -94,6 +94,99 @@ static __global__ void __launch_bounds__(256) k_cycle_energies(int N, const doub
     cycle_energy_column<D>(v, N, threadIdx.x % WAVE, x1, xP, k, pbc != 0, L, E, ET);
 }
 
+// ---- double-double helpers (error-free transformations; no contraction) ----
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD dd_two_sum(double a, double b) {
+#pragma clang fp contract(off)
+    const double s = a + b;
+    const double bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+#pragma clang fp contract(off)
+    const DD s = dd_two_sum(a.hi, b.hi);
+    const double lo = s.lo + (a.lo + b.lo);
+    const double hi = s.hi + lo;
+    return DD{hi, lo - (hi - s.hi)};
+}
+// (a - b) rounded to double
+__device__ __forceinline__ double dd_diff(double ahi, double alo, double bhi, double blo) {
+#pragma clang fp contract(off)
+    const DD s = dd_two_sum(ahi, -bhi);
+    return s.hi + (s.lo + (alo - blo));
+}
+
+// ---- cycle energies from link prefix sums ----
+// With link_j = |x1[j+1] - xP[j]|^2 (min-imaged) and S_v = sum_{j<v} link_j,
+//     ecyc(u,v) = k/2 (S_v - S_u + |x1[u] - xP[v]|^2),
+// the closed form of the reference's recurrence (quadratic_bosonic_exchange.cpp
+// :45-60).  S is kept as a double-double so S_v - S_u keeps full precision
+// however long the ring: any ecyc is a few flops from O(N) data and the
+// N(N+1)/2 triangle is never stored on the step path.
+struct Ring {
+    const double* x1;    // [D][N] bead 1
+    const double* xP;    // [D][N] bead P
+    const double* Shi;   // [N] prefix sums (high and low parts)
+    const double* Slo;
+    double halfk, L, invL;
+    int pbc;
+};
+
+// Minimum image for a squared distance (the image choice at |dx| = L/2 does
+// not change the square); one rounding less than the reference's
+// dx - L floor(dx/L + 1/2) (bosonic_exchange_base.cpp:30-42).
+__device__ __forceinline__ double ring_image(double dx, const Ring& r) {
+    return r.pbc ? fma(-r.L, rint(dx * r.invL), dx) : dx;
+}
+
+// ecyc(u,v) from the min-imaged displacement x1[u] - xP[v] (either sign).
+template <int D>
+__device__ __forceinline__ double ring_ecyc(const Ring& r, const double (&d)[D], double shu, double slu, double shv,
+                                            double slv) {
+    return r.halfk * (dd_diff(shv, slv, shu, slu) + norm2<D>(d));
+}
+
+// S_0..S_{N-1} of the ring by one wave (chunks of ceil(N/64) per lane, then a
+// scan of the chunk totals across lanes); ring.x1 / ring.xP readable, Shi /
+// Slo written (LDS or global).
+template <int D>
+__device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, double* Shi, double* Slo) {
+    const int c = (N + WAVE - 1) / WAVE;
+    const int v0 = lane * c;
+    DD acc{0.0, 0.0};
+    for (int t = 0; t < c; ++t) {
+        const int v = v0 + t;
+        if (v < N) {
+            Shi[v] = acc.hi;
+            Slo[v] = acc.lo;
+            if (v < N - 1) {
+                double d[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) d[a] = ring_image(r.x1[a * N + v + 1] - r.xP[a * N + v], r);
+                acc = dd_add(acc, DD{norm2<D>(d), 0.0});
+            }
+        }
+    }
+    DD inc = acc;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const DD o{__shfl_up(inc.hi, off, WAVE), __shfl_up(inc.lo, off, WAVE)};
+        if (lane >= off) inc = dd_add(o, inc);
+    }
+    DD ex{__shfl_up(inc.hi, 1, WAVE), __shfl_up(inc.lo, 1, WAVE)};
+    if (lane == 0) ex = DD{0.0, 0.0};
+    for (int t = 0; t < c; ++t) {
+        const int v = v0 + t;
+        if (v < N) {
+            const DD sv = dd_add(ex, DD{Shi[v], Slo[v]});
+            Shi[v] = sv.hi;
+            Slo[v] = sv.lo;
+        }
+    }
+}
+
 // Shared-memory workspace of one workgroup running one recursion.
 struct RecShared {
     double* U;         // [N+1]
@@ -101,49 +194,102 @@ struct RecShared {
     double* R;         // [REC_B] shifts of the block's targets
     double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min / sum
     double* a;         // [REC_B*REC_B] in-block weights, then coefficients, a[j*REC_B + t]
+    double* x1;        // [D][N] bead 1
+    double* xP;        // [D][N] bead P
+    double* Shi;       // [N] link prefix sums
+    double* Slo;       // [N]
 };
 
-__host__ __device__ __forceinline__ size_t rec_shared_doubles(int N) {
-    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
+__host__ __device__ __forceinline__ size_t rec_shared_doubles(int N, int D) {
+    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B +
+           (size_t)(2 * D + 2) * N;
 }
 
-__device__ __forceinline__ RecShared rec_carve(double* base, int N) {
+__device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
     RecShared s;
     double* p = base;
     s.U = p; p += N + 1;
     s.lg = p; p += N + 1;
     s.R = p; p += REC_B;
     s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
-    s.a = p;
+    s.a = p; p += (size_t)REC_B * REC_B;
+    s.x1 = p; p += (size_t)D * N;
+    s.xP = p; p += (size_t)D * N;
+    s.Shi = p; p += N;
+    s.Slo = p;
     return s;
 }
 
-// w(n,q) of direction dir (see file header).  The forward direction reads
-// the row-major copy ET (row q, contiguous in n), the backward one reads E
-// (column N-1-q, contiguous in n): consecutive targets hit consecutive
-// addresses in both.
-template <int DIR>
-__device__ __forceinline__ double rec_w(int N, const double* __restrict__ E, const double* __restrict__ ET, double bt,
-                                        const double* lg, int n, int q) {
-    if constexpr (DIR == 0) {
-        return bt * ET[et_row(q, N) + (n - 1 - q)] + lg[n];
-    } else {
-        const int p = N - 1 - q;
-        return bt * E[(size_t)p * (p + 1) / 2 + (N - n)] + lg[N - q];
+// Copies beads 1 and P into the workspace and builds the prefix sums there
+// (whole workgroup); optionally also writes the sums to global memory.
+template <int D>
+__device__ __forceinline__ Ring ring_stage(const RecShared& sh, int N, const double* x1, const double* xP, double k,
+                                           bool pbc, double L, double* Shi_out, double* Slo_out) {
+    for (int e = threadIdx.x; e < D * N; e += blockDim.x) {
+        sh.x1[e] = x1[e];
+        sh.xP[e] = xP[e];
     }
+    __syncthreads();
+    Ring r{sh.x1, sh.xP, sh.Shi, sh.Slo, 0.5 * k, L, 1.0 / L, pbc ? 1 : 0};
+    if (threadIdx.x < WAVE) {
+        ring_prefix<D>(r, N, threadIdx.x, sh.Shi, sh.Slo);
+        if (Shi_out) {
+            for (int v = threadIdx.x; v < N; v += WAVE) {
+                Shi_out[v] = sh.Shi[v];
+                Slo_out[v] = sh.Slo[v];
+            }
+        }
+    }
+    __syncthreads();
+    return r;
+}
+
+// The target side of w(n, q) held by one lane: forward, n-1 is the cycle's
+// last particle v (xP[v], S_v) and ln n is added; backward, N-n is its first
+// particle u (x1[u], S_u).
+template <int D>
+struct RecTarget {
+    double x[D];
+    double shi, slo, lg;
+};
+
+template <int DIR, int D>
+__device__ __forceinline__ RecTarget<D> rec_target(const RecShared& sh, int N, int n) {
+    RecTarget<D> t;
+    const int idx = DIR == 0 ? n - 1 : N - n;
+    const double* xs = DIR == 0 ? sh.xP : sh.x1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) t.x[a] = xs[a * N + idx];
+    t.shi = sh.Shi[idx];
+    t.slo = sh.Slo[idx];
+    t.lg = DIR == 0 ? sh.lg[n] : 0.0;
+    return t;
+}
+
+// w(n,q) (see file header) for the lane's target and source q.
+template <int DIR, int D>
+__device__ __forceinline__ double rec_w(const RecShared& sh, const Ring& r, double bt, int N, const RecTarget<D>& t,
+                                        int q) {
+    const int idx = DIR == 0 ? q : N - 1 - q;
+    const double* xs = DIR == 0 ? sh.x1 : sh.xP;
+    double d[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) d[a] = ring_image(xs[a * N + idx] - t.x[a], r);
+    const double e = DIR == 0 ? ring_ecyc<D>(r, d, sh.Shi[idx], sh.Slo[idx], t.shi, t.slo)
+                              : ring_ecyc<D>(r, d, t.shi, t.slo, sh.Shi[idx], sh.Slo[idx]);
+    return bt * e + (DIR == 0 ? t.lg : sh.lg[N - q]);
 }
 
 // One recursion (DIR 0: forward V, DIR 1: backward Vb) for one ring, run by
-// the whole workgroup (64..1024 threads).  Lane t of every wave serves target
-// n0+t of the current block; the waves split the sources of earlier blocks
-// and keep UNROLL loads in flight.  Writes V[0..N] and Vb[0] (DIR 0) or
-// Vb[1..N] (DIR 1); sets *status to 1 when a potential is not finite (the
-// reference throws overflow_error, quadratic_bosonic_exchange.cpp:92-97,
-// 119-124).
-template <int DIR, int UNROLL = REC_UNROLL>
-__device__ void exchange_recursion(int N, double bt, const double* __restrict__ E, const double* __restrict__ ET,
-                                   double* __restrict__ V, double* __restrict__ Vb, int* __restrict__ status,
-                                   RecShared sh) {
+// the whole workgroup (64..1024 threads) on the staged ring (ring_stage).
+// Lane t of every wave serves target n0+t of the current block; the waves
+// split the sources of earlier blocks and keep UNROLL evaluations in flight.
+// Writes V[0..N] and Vb[0] (DIR 0) or Vb[1..N] (DIR 1); sets *status to 1
+// when a potential is not finite (the reference throws overflow_error,
+// quadratic_bosonic_exchange.cpp:92-97, 119-124).
+template <int DIR, int D, int UNROLL = REC_UNROLL>
+__device__ void exchange_recursion(int N, double bt, const Ring& r, double* __restrict__ V, double* __restrict__ Vb,
+                                   int* __restrict__ status, RecShared sh) {
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int nwave = min(nthr / WAVE, REC_MAXWAVES);
@@ -155,11 +301,11 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
     __syncthreads();
     for (int n0 = 1; n0 <= ntarget; n0 += REC_B) {
         const int nb = min(REC_B, ntarget - n0 + 1);
+        const int tn = n0 + min(lane, nb - 1);   // the lane's target (clamped)
+        const RecTarget<D> tg = rec_target<DIR, D>(sh, N, tn);
         // (0) in-block weights w(n0+t, n0+j), j < t, staged as a[j][t]
-        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
-            const int j = idx / REC_B, t = idx % REC_B;
-            sh.a[idx] = (t < nb && j < t) ? rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + t, n0 + j) : 0.0;
-        }
+        for (int j = wave; j < REC_B; j += nthr / WAVE)
+            sh.a[j * REC_B + lane] = (lane < nb && j < lane) ? rec_w<DIR, D>(sh, r, bt, N, tg, n0 + j) : 0.0;
         // (1a) per-wave partial min over this wave's share of earlier sources
         const int chunk = (n0 + nwave - 1) / nwave;
         const int qlo = min(n0, wave * chunk), qhi = min(n0, qlo + chunk);
@@ -171,11 +317,11 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
                 for (; q + UNROLL <= qhi; q += UNROLL) {
                     double w[UNROLL];
 #pragma unroll
-                    for (int r = 0; r < UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+                    for (int u = 0; u < UNROLL; ++u) w[u] = rec_w<DIR, D>(sh, r, bt, N, tg, q + u);
 #pragma unroll
-                    for (int r = 0; r < UNROLL; ++r) m = fmin(m, w[r] + sh.U[q + r]);
+                    for (int u = 0; u < UNROLL; ++u) m = fmin(m, w[u] + sh.U[q + u]);
                 }
-                for (; q < qhi; ++q) m = fmin(m, rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q]);
+                for (; q < qhi; ++q) m = fmin(m, rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q]);
             }
             sh.part[wave * REC_B + lane] = m;
         }
@@ -183,17 +329,17 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
         // (1b) in-block min-plus on wave 0
         if (wave == 0) {
             const int t = lane;
-            double r = INFINITY;
-            for (int w = 0; w < nwave; ++w) r = fmin(r, sh.part[w * REC_B + t]);
-            if (t >= nb) r = INFINITY;
+            double rr = INFINITY;
+            for (int w = 0; w < nwave; ++w) rr = fmin(rr, sh.part[w * REC_B + t]);
+            if (t >= nb) rr = INFINITY;
             double wnext = sh.a[t];
             for (int j = 0; j + 1 < nb; ++j) {
                 const double wj = wnext;
                 wnext = sh.a[(j + 1) * REC_B + t];
-                const double Rj = wave_bcast_uniform(r, j);
-                if (t > j) r = fmin(r, wj + Rj);
+                const double Rj = wave_bcast_uniform(rr, j);
+                if (t > j) rr = fmin(rr, wj + Rj);
             }
-            sh.R[t] = r;
+            sh.R[t] = rr;
         }
         __syncthreads();
         // (2a) per-wave partial sums over earlier sources
@@ -205,11 +351,11 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
                 for (; q + UNROLL <= qhi; q += UNROLL) {
                     double w[UNROLL];
 #pragma unroll
-                    for (int r = 0; r < UNROLL; ++r) w[r] = rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q + r);
+                    for (int u = 0; u < UNROLL; ++u) w[u] = rec_w<DIR, D>(sh, r, bt, N, tg, q + u);
 #pragma unroll
-                    for (int r = 0; r < UNROLL; ++r) s += exp(-(w[r] + sh.U[q + r] - Rn));
+                    for (int u = 0; u < UNROLL; ++u) s += exp(-(w[u] + sh.U[q + u] - Rn));
                 }
-                for (; q < qhi; ++q) s += exp(-(rec_w<DIR>(N, E, ET, bt, sh.lg, n0 + lane, q) + sh.U[q] - Rn));
+                for (; q < qhi; ++q) s += exp(-(rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q] - Rn));
             }
             sh.part[wave * REC_B + lane] = s;
         }
@@ -252,14 +398,21 @@ __device__ void exchange_recursion(int N, double bt, const double* __restrict__ 
     __syncthreads();
 }
 
-// Block 0 runs the forward recursion, block 1 the backward one.
-static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, double bt, const double* __restrict__ E,
-                                                                     const double* __restrict__ ET,
+// Block 0 runs the forward recursion (and writes the ring's prefix sums for
+// the exterior forces), block 1 the backward one.  x1, xP: SoA [a][i].
+template <int D>
+static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, double bt, double k, int pbc, double L,
+                                                                     const double* __restrict__ x1,
+                                                                     const double* __restrict__ xP,
                                                                      double* __restrict__ V, double* __restrict__ Vb,
+                                                                     double* __restrict__ Shi, double* __restrict__ Slo,
                                                                      int* __restrict__ status) {
     extern __shared__ double smem[];
-    if (blockIdx.x == 0) exchange_recursion<0>(N, bt, E, ET, V, Vb, status, rec_carve(smem, N));
-    else exchange_recursion<1>(N, bt, E, ET, V, Vb, status, rec_carve(smem, N));
+    const RecShared sh = rec_carve(smem, N, D);
+    const bool fwd = blockIdx.x == 0;
+    const Ring r = ring_stage<D>(sh, N, x1, xP, k, pbc != 0, L, fwd ? Shi : nullptr, fwd ? Slo : nullptr);
+    if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh);
+    else exchange_recursion<1, D>(N, bt, r, V, Vb, status, sh);
 }
 
 // Connection probability P[l][u] (quadratic_bosonic_exchange.cpp:142-157).
@@ -289,43 +442,51 @@ static __global__ void __launch_bounds__(256) k_exchange_probabilities(int N, do
 // (which=1), one wave per (which, l):
 //  bead 1: F[l] = k [ sum_{j>=max(0,l-1)} P[j][l] (xP[j]-x1[l]) + (x2[l]-x1[l]) ]   (:188-215)
 //  bead P: F[l] = k [ sum_{j<=min(l+1,N-1)} P[l][j] (x1[j]-xP[l]) + (xPm1[l]-xP[l]) ] (:159-186)
-// Probabilities are recomputed from V, Vb and E (ET required).  With grad_only the
-// interior link is left out and the result is -dV_B/dx (the adjoint of V_B
-// alone, used by the autograd harness).
+// Probabilities are recomputed from V, Vb and the ring (conn_prob's
+// expression, :142-157); the displacement of a term also gives its cycle
+// energy (ecyc(l,j) for bead 1, ecyc(j,l) for bead P).  x2 / xPm1 hold the
+// interior neighbours; with grad_only the interior link is left out and the
+// result is -dV_B/dx (the adjoint of V_B alone, used by the autograd harness).
 template <int D, int JUNROLL = 4>
-__device__ __forceinline__ void exterior_force(int which, int l, int N, int lane, double bt, double k, bool pbc,
-                                               double L, const double* __restrict__ E, const double* __restrict__ ET,
-                                               const double* __restrict__ V,
-                                               const double* __restrict__ Vb, const double* __restrict__ x1,
+__device__ __forceinline__ void exterior_force(int which, int l, int N, int lane, double bt, double k, const Ring& r,
+                                               const double* __restrict__ V, const double* __restrict__ Vb,
                                                const double* __restrict__ x2, const double* __restrict__ xPm1,
-                                               const double* __restrict__ xP, bool grad_only, double (&F)[D]) {
-    const double* self = which == 0 ? x1 : xP;
-    const double* other = which == 0 ? xP : x1;
+                                               bool grad_only, double (&F)[D]) {
+    const double* self = which == 0 ? r.x1 : r.xP;
+    const double* other = which == 0 ? r.xP : r.x1;
     double xs[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) xs[a] = self[a * N + l];
+    const double shl = r.Shi[l], sll = r.Slo[l];
     double S[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) S[a] = 0.0;
     const int jlo = which == 0 ? max(0, l - 1) : 0;
     const int jhi = which == 0 ? N - 1 : min(l + 1, N - 1);
-    // P[row][col] (conn_prob's expression) with (row, col) = (j, l) for bead
-    // 1 and (l, j) for bead P; the cycle energy ecyc(col, row) it needs is
-    // ebase[j] in both cases (row l of ET, or row l of the packed E).
-    const double* ebase = which == 0 ? ET + et_row(l, N) - l : E + (size_t)l * (l + 1) / 2;
     const double VN = V[N];
 #pragma unroll JUNROLL
     for (int j = jlo + lane; j <= jhi; j += WAVE) {
         const int row = which == 0 ? j : l, col = which == 0 ? l : j;
+        double d[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) d[a] = ring_image(other[a * N + j] - xs[a], r);
         const bool link = col == row + 1 && row < N - 1;
         double pr = 0.0;
         if (link || col <= row) {
-            const double arg = link ? V[row + 1] + Vb[row + 1] - VN : V[col] + ebase[j] + Vb[row + 1] - VN;
+            double arg;
+            if (link) {
+                arg = V[row + 1] + Vb[row + 1] - VN;
+            } else {
+                // ecyc(col, row): u = col, v = row
+                const double e = which == 0 ? ring_ecyc<D>(r, d, shl, sll, r.Shi[j], r.Slo[j])
+                                            : ring_ecyc<D>(r, d, r.Shi[j], r.Slo[j], shl, sll);
+                arg = V[col] + e + Vb[row + 1] - VN;
+            }
             const double ex = exp(-bt * arg);
             pr = link ? 1.0 - ex : 1.0 / (row + 1) * ex;
         }
 #pragma unroll
-        for (int a = 0; a < D; ++a) S[a] += pr * disp(xs[a], other[a * N + j], pbc, L);
+        for (int a = 0; a < D; ++a) S[a] += pr * d[a];
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) S[a] = wave_sum(S[a]);
@@ -333,16 +494,18 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         double s = S[a];
-        if (!grad_only) s += disp(xs[a], link[a * N + l], pbc, L);
+        if (!grad_only) s += ring_image(link[a * N + l] - xs[a], r);
         F[a] = s * k;
     }
 }
 
 // Adds the exterior spring forces into f for beads 1 and P:
 // f = F_exchange + f_phys (simulation.cpp:363-373: spring + physical).
+// Shi/Slo: the ring's prefix sums from k_exchange_recursions.
 template <int D>
 static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, double bt, double k, int pbc, double L,
-                                                         const double* __restrict__ E, const double* __restrict__ ET,
+                                                         const double* __restrict__ Shi,
+                                                         const double* __restrict__ Slo,
                                                          const double* __restrict__ V,
                                                          const double* __restrict__ Vb, const double* __restrict__ x,
                                                          double* __restrict__ f, double* __restrict__ fex) {
@@ -351,12 +514,11 @@ static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, do
     const int which = task / N, l = task % N;
     const int lane = threadIdx.x % WAVE;
     const size_t bs = (size_t)D * N;
-    const double* x1 = x;
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
-    const double* xP = x + bs * (P - 1);
+    const Ring r{x, x + bs * (P - 1), Shi, Slo, 0.5 * k, L, 1.0 / L, pbc};
     double F[D];
-    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, ET, V, Vb, x1, x2, xPm1, xP, false, F);
+    exterior_force<D>(which, l, N, lane, bt, k, r, V, Vb, x2, xPm1, false, F);
     if (lane == 0) {
         const size_t b = which == 0 ? 0 : (size_t)(P - 1);
 #pragma unroll
@@ -372,7 +534,8 @@ static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, do
 // given separately in SoA [a][i]; out[which][a][i].
 template <int D>
 static __global__ void __launch_bounds__(256) k_exchange_vb_grad(int N, double bt, double k, int pbc, double L,
-                                                          const double* __restrict__ E, const double* __restrict__ ET,
+                                                          const double* __restrict__ Shi,
+                                                          const double* __restrict__ Slo,
                                                           const double* __restrict__ V,
                                                           const double* __restrict__ Vb, const double* __restrict__ x1,
                                                           const double* __restrict__ xP, double* __restrict__ out) {
@@ -380,8 +543,9 @@ static __global__ void __launch_bounds__(256) k_exchange_vb_grad(int N, double b
     if (task >= 2 * N) return;
     const int which = task / N, l = task % N;
     const int lane = threadIdx.x % WAVE;
+    const Ring r{x1, xP, Shi, Slo, 0.5 * k, L, 1.0 / L, pbc};
     double F[D];
-    exterior_force<D>(which, l, N, lane, bt, k, pbc != 0, L, E, ET, V, Vb, x1, x1, xP, xP, true, F);
+    exterior_force<D>(which, l, N, lane, bt, k, r, V, Vb, x1, xP, true, F);
     if (lane == 0) {
 #pragma unroll
         for (int a = 0; a < D; ++a) out[((size_t)which * D + a) * N + l] = F[a];

@@ -23,7 +23,7 @@ struct pimdb_batch {
     int R = 0, D = 3, kind = 0;
     int threads = 1024, resident = 0;   // resident workgroups (0: all CUs)
     int cus = 0;
-    size_t nx = 0, nE = 0, nV = 0, nC = 0;
+    size_t nx = 0, nV = 0, nC = 0;
     size_t lds = 0;
     BatchPtrs bp{};
     RepDesc* d_desc = nullptr;
@@ -209,7 +209,6 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         d.ext = dv.ext; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
         d.seed = p.seed; d.rid = 0;   // noise stream keyed by the seed, as in the single-system form
         d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
-        d.off_E = b->nE; b->nE += (size_t)p.natoms * (p.natoms + 1) / 2;
         d.off_V = b->nV; b->nV += (size_t)p.natoms + 1;
         d.off_C = b->nC; b->nC += (size_t)p.nbeads * p.nbeads;
         b->desc.push_back(d);
@@ -218,7 +217,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     size_t need = 0;
     for (int r = 0; r < nrep; ++r) {
         const pimdb_params& p = reps[r];
-        need = std::max(need, rec_shared_doubles(p.natoms));
+        need = std::max(need, rec_shared_doubles(p.natoms, p.ndim));
         need = std::max(need, batch_nm_lds_doubles(p.nbeads));
     }
     b->lds = sizeof(double) * (16 + need);
@@ -232,7 +231,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     };
     double *Ct = nullptr, *CTt = nullptr;
     alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
-    alloc(&b->bp.E, b->nE); alloc(&b->bp.ET, b->nE); alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
+    alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
     alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
     b->bp.Ct = Ct;
     b->bp.CTt = CTt;
@@ -275,7 +274,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->s) (void)hipStreamSynchronize(b->s);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.E, b->bp.ET, b->bp.V, b->bp.Vb,
+    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.Vb,
                       const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
     for (double* p : bufs)
         if (p) (void)hipFree(p);

@@ -4,7 +4,7 @@
 // One MD step (Simulation::run body, src/simulation.cpp:246-259) is
 //   K1 k_momenta      O-step (+ deferred zeroMomentum of the previous step)
 //   K2 k_kick_drift   zeroMomentum, B, A          (or k_nm_propagate)
-//   K3 forces         s1: k_cycle_energies -> k_exchange_recursions
+//   K3 forces         s1: k_exchange_recursions (cycle energies on the fly)
 //                     s0: k_bead_forces (all beads, all pairs)
 //                     s0 after join: k_exchange_forces (beads 1 and P)
 //   K4 k_momenta      B, O-step, partial sums for zeroMomentum
@@ -36,6 +36,7 @@ struct pimdb_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, t0 = nullptr, t1 = nullptr;
     double *x = nullptr, *p = nullptr, *f = nullptr, *fext = nullptr;
     double *E = nullptr, *ET = nullptr, *V = nullptr, *Vb = nullptr, *prob = nullptr, *fex = nullptr;
+    double *Shi = nullptr, *Slo = nullptr;   // link prefix sums of the ring (exterior forces)
     double *C = nullptr, *CT = nullptr, *rot = nullptr, *com0 = nullptr, *com1 = nullptr;
     double *noise = nullptr, *obs = nullptr;
     size_t noise_cap = 0;
@@ -84,20 +85,34 @@ static void launch_bead_forces(pimdb_ctx* ctx, hipStream_t s) {
     }
 }
 
+// Both recursions (cycle energies recomputed from the ring's prefix sums,
+// which block 0 also leaves in Shi/Slo for the exterior forces).
 static void launch_exchange_forward(pimdb_ctx* ctx, hipStream_t s) {
+    const int N = ctx->N, P = ctx->P, D = ctx->D;
+    const double* x1 = ctx->x;
+    const double* xP = ctx->x + (size_t)(P - 1) * D * N;
+    const size_t lds = rec_lds_bytes(N, D);
+    switch (D) {
+    case 1: hipLaunchKernelGGL(k_exchange_recursions<1>, dim3(2), dim3(1024), lds, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status); break;
+    case 2: hipLaunchKernelGGL(k_exchange_recursions<2>, dim3(2), dim3(1024), lds, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status); break;
+    default: hipLaunchKernelGGL(k_exchange_recursions<3>, dim3(2), dim3(1024), lds, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status); break;
+    }
+}
+
+// The stored triangle E (and its row-major copy), for the outputs and
+// estimators that take E itself.
+static void launch_cycle_energies_ctx(pimdb_ctx* ctx, hipStream_t s) {
     const int N = ctx->N, P = ctx->P;
     const double* x1 = ctx->x;
     const double* xP = ctx->x + (size_t)(P - 1) * ctx->D * N;
     launch_cycle_energies(ctx->D, N, x1, xP, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, ctx->ET, s);
-    hipLaunchKernelGGL(k_exchange_recursions, dim3(2), dim3(1024), rec_lds_bytes(N), s, N, ctx->dv.bt, ctx->E,
-                       ctx->ET, ctx->V, ctx->Vb, ctx->status);
 }
 
 template <int D>
 static void launch_exchange_forces_d(pimdb_ctx* ctx, hipStream_t s, double* f, double* fex) {
     const int N = ctx->N;
     hipLaunchKernelGGL((k_exchange_forces<D>), dim3((2 * N + 3) / 4), dim3(256), 0, s, N, ctx->P, ctx->dv.bt,
-                       ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->x, f, fex);
+                       ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->Shi, ctx->Slo, ctx->V, ctx->Vb, ctx->x, f, fex);
 }
 
 static void launch_exchange_forces(pimdb_ctx* ctx, hipStream_t s, double* f, double* fex) {
@@ -260,6 +275,8 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     alloc(&ctx->ET, sizeof(double) * ((size_t)N * (N + 1) / 2));
     alloc(&ctx->V, sizeof(double) * (N + 1));
     alloc(&ctx->Vb, sizeof(double) * (N + 1));
+    alloc(&ctx->Shi, sizeof(double) * N);
+    alloc(&ctx->Slo, sizeof(double) * N);
     alloc(&ctx->prob, sizeof(double) * (size_t)N * N);
     alloc(&ctx->fex, sizeof(double) * 2 * (size_t)D * N);
     alloc(&ctx->C, sizeof(double) * (size_t)P * P);
@@ -288,7 +305,7 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
             hipMemcpy(ctx->rot, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice) != hipSuccess)
             rc = ctx->fail(PIMDB_E_HIP, "upload of normal-mode tables failed");
     }
-    if (!rc && ctx->dv.bosonic && rec_lds_bytes(N) > device_lds_limit())
+    if (!rc && ctx->dv.bosonic && rec_lds_bytes(N, D) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the exchange workgroup");
     if (!rc && sizeof(double) * ((size_t)D * N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the force workgroup");
@@ -308,7 +325,7 @@ void pimdb_ctx_destroy(pimdb_ctx* ctx) {
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     double* bufs[] = {ctx->x, ctx->p, ctx->f, ctx->fext, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->prob, ctx->fex,
-                      ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs};
+                      ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs, ctx->Shi, ctx->Slo};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->status) (void)hipFree(ctx->status);
@@ -367,6 +384,7 @@ int pimdb_exchange_forward(pimdb_ctx* ctx, double* VB, double* V, double* Vb, do
     if (!ctx->dv.bosonic) return ctx->fail(PIMDB_E_STATE, "exchange requires bosonic=true and nbeads > 1");
     const int N = ctx->N;
     launch_exchange_forward(ctx, ctx->s0);
+    if (prob || E) launch_cycle_energies_ctx(ctx, ctx->s0);
     if (prob) {
         const size_t nn = (size_t)N * N;
         hipLaunchKernelGGL(k_exchange_probabilities, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->s0, N,
@@ -474,6 +492,7 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
     launch_bead_energies(D, ctx->prm, ctx->dv, ctx->x, epart, ctx->s0);
     if (ctx->dv.bosonic) {
         launch_exchange_forward(ctx, ctx->s0);
+        launch_cycle_energies_ctx(ctx, ctx->s0);
         hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
                            ctx->dv.bt, ctx->E, ctx->V, prim);
     }
@@ -496,6 +515,7 @@ int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinc
     const int nt = (N + FORCE_TILE - 1) / FORCE_TILE;
     double* out = ctx->obs + 2 * (size_t)P + 3 * (size_t)P * nt;   // [3], as pimdb_observables
     launch_exchange_forward(ctx, ctx->s0);
+    launch_cycle_energies_ctx(ctx, ctx->s0);
     hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
                        ctx->dv.bt, ctx->E, ctx->V, out);
     CK(hipGetLastError());

@@ -13,14 +13,17 @@ using namespace pimdb;
 struct pimdb_xplan {
     int N = 0, D = 0, pbc = 0;
     double bt = 0.0, k = 0.0, L = 0.0;
-    double *E = nullptr, *ET = nullptr, *V = nullptr, *Vb = nullptr;
+    double *V = nullptr, *Vb = nullptr, *Shi = nullptr, *Slo = nullptr;
     int* status = nullptr;
 };
 
 static int xplan_forward_internal(pimdb_xplan* pl, const double* x1, const double* xP, hipStream_t s) {
-    launch_cycle_energies(pl->D, pl->N, x1, xP, pl->k, pl->pbc, pl->L, pl->E, pl->ET, s);
-    hipLaunchKernelGGL(k_exchange_recursions, dim3(2), dim3(1024), rec_lds_bytes(pl->N), s, pl->N, pl->bt, pl->E,
-                       pl->ET, pl->V, pl->Vb, pl->status);
+    const size_t lds = rec_lds_bytes(pl->N, pl->D);
+    switch (pl->D) {
+    case 1: hipLaunchKernelGGL(k_exchange_recursions<1>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
+    case 2: hipLaunchKernelGGL(k_exchange_recursions<2>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
+    default: hipLaunchKernelGGL(k_exchange_recursions<3>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
+    }
     if (hipGetLastError() != hipSuccess) return PIMDB_E_HIP;
     int st = 0;
     if (hipMemcpyAsync(&st, pl->status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return PIMDB_E_HIP;
@@ -41,12 +44,11 @@ int pimdb_xplan_create(int natoms, int ndim, double beta_t, double spring_k, int
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PIMDB_E_HIP;
-    if (rec_lds_bytes(natoms) > device_lds_limit()) return PIMDB_E_ARG;
+    if (rec_lds_bytes(natoms, ndim) > device_lds_limit()) return PIMDB_E_ARG;
     pimdb_xplan* pl = new pimdb_xplan();
     pl->N = natoms; pl->D = ndim; pl->pbc = pbc; pl->bt = beta_t; pl->k = spring_k; pl->L = size;
-    const size_t ne = (size_t)natoms * (natoms + 1) / 2;
-    if (hipMalloc((void**)&pl->E, ne * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&pl->ET, ne * sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void**)&pl->Shi, natoms * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&pl->Slo, natoms * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->V, (natoms + 1) * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->Vb, (natoms + 1) * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->status, sizeof(int)) != hipSuccess || hipMemset(pl->status, 0, sizeof(int)) != hipSuccess) {
@@ -60,8 +62,8 @@ int pimdb_xplan_create(int natoms, int ndim, double beta_t, double spring_k, int
 void pimdb_xplan_destroy(pimdb_xplan* pl) {
     if (!pl) return;
     (void)hipDeviceSynchronize();
-    if (pl->E) (void)hipFree(pl->E);
-    if (pl->ET) (void)hipFree(pl->ET);
+    if (pl->Shi) (void)hipFree(pl->Shi);
+    if (pl->Slo) (void)hipFree(pl->Slo);
     if (pl->V) (void)hipFree(pl->V);
     if (pl->Vb) (void)hipFree(pl->Vb);
     if (pl->status) (void)hipFree(pl->status);
@@ -89,9 +91,9 @@ int pimdb_xplan_vb_grad(pimdb_xplan* pl, const double* x1, const double* xP, dou
     const int N = pl->N;
     dim3 grid((2 * N + 3) / 4), block(256);
     switch (pl->D) {
-    case 1: hipLaunchKernelGGL(k_exchange_vb_grad<1>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->E, pl->ET, pl->V, pl->Vb, x1, xP, grad); break;
-    case 2: hipLaunchKernelGGL(k_exchange_vb_grad<2>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->E, pl->ET, pl->V, pl->Vb, x1, xP, grad); break;
-    default: hipLaunchKernelGGL(k_exchange_vb_grad<3>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->E, pl->ET, pl->V, pl->Vb, x1, xP, grad); break;
+    case 1: hipLaunchKernelGGL(k_exchange_vb_grad<1>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->Shi, pl->Slo, pl->V, pl->Vb, x1, xP, grad); break;
+    case 2: hipLaunchKernelGGL(k_exchange_vb_grad<2>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->Shi, pl->Slo, pl->V, pl->Vb, x1, xP, grad); break;
+    default: hipLaunchKernelGGL(k_exchange_vb_grad<3>, grid, block, 0, s, N, pl->bt, pl->k, pl->pbc, pl->L, pl->Shi, pl->Slo, pl->V, pl->Vb, x1, xP, grad); break;
     }
     if (hipGetLastError() != hipSuccess) return PIMDB_E_HIP;
     return hipStreamSynchronize(s) == hipSuccess ? PIMDB_OK : PIMDB_E_HIP;
