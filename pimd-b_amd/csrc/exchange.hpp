@@ -7,8 +7,12 @@
 //
 // Storage: bead-major SoA positions, x[(b*D + a)*N + i].  The cycle energy of
 // the cycle through particles u..v (u <= v) is ecyc(u,v) = E^(v+1, v-u+1) of
-// the reference and lives at E[v(v+1)/2 + u]: this is exactly the reference's
-// packed E_kn triangle (E(m,k) at m(m+1)/2 - k), so column v is contiguous.
+// the reference.  On the step path it is recomputed wherever it is needed
+// from beads 1 and P and the double-double prefix sums of the ring's links
+// (Ring, below: O(N) data, staged in LDS).  Only when a caller asks for the
+// energies themselves (or for the observables) is the triangle materialised,
+// at E[v(v+1)/2 + u]: exactly the reference's packed E_kn triangle (E(m,k) at
+// m(m+1)/2 - k), so column v is contiguous.
 //
 // Recursions.  Both recursions have the form
 //     U_n = -ln sum_{q<n} exp(-(w(n,q) + U_q)),  U_0 = 0,
