@@ -135,10 +135,16 @@ int pimdb_graph_capture(pimdb_ctx* ctx);
 int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_exchange, double* ms_step);
 
 /* Micro-benchmarks for the roofline: FP64 FMA rate (TFLOP/s), FP64 exp
- * throughput of the device math library (Gexp/s) and HBM copy bandwidth
+ * throughput of the pair kernels' exp (Gexp/s) and HBM copy bandwidth
  * (GB/s, read + write, over copy_bytes; <= 0 means 1 GiB).  Any output may
  * be NULL.  No reference counterpart. */
 int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gbs, long long copy_bytes);
+
+/* Evaluates the pair kernels' device elementary functions on n host values:
+ * fn 0 = e^x (exp_minimax), fn 1 = 1/sqrt(x) (rsqrt_refined), y = f(x).
+ * A test hook for their accuracy.  No reference counterpart (the reference
+ * calls the C library's exp and sqrt, aziz.cpp:56-106). */
+int pimdb_math_probe(int fn, const double* x, double* y, long long n);
 
 /* ---------------------------------------------------------------------- */
 /* Exchange on caller-provided device arrays (the autograd harness).       */

@@ -70,12 +70,6 @@ __device__ __forceinline__ double norm2(const double (&d)[D]) {
     return s;
 }
 
-// e^x for x <= 709 (the pair potentials evaluate it at x <= 0), as 2^k e^r
-// with k = rint(x / ln 2) and r = x - k ln 2 reduced with the two-part ln 2 of
-// Cody and Waite (1980), |r| <= ln(2)/2; e^r is its Taylor series through
-// r^13/13! (the first omitted term is below 5e-18 relative).  Relative error
-// within about 2 ulp of the correctly rounded value; arguments below -746
-// give 0, NaN stays NaN.
 // a * b + c with c in a scalar register.  The compiler turns p = fma(p, r, c)
 // with a loop-invariant c into a copy of c plus v_fmac; a three-operand
 // v_fma_f64 with the constant in an SGPR needs neither.
@@ -85,25 +79,31 @@ __device__ __forceinline__ double fma_sconst(double a, double b, double c) {
     return d;
 }
 
-__device__ __forceinline__ double exp_taylor(double x) {
+// e^x for x <= 709 (the pair potentials evaluate it at x <= 0), as 2^k e^r
+// with k = rint(x / ln 2) and r = x - k ln 2 reduced with the two-part ln 2 of
+// Cody and Waite (1980), |r| <= ln(2)/2.  e^r is the degree-11 polynomial of
+// best relative approximation on that interval (Remez exchange; coefficients
+// from gen_exp_minimax.py beside this file): its approximation error is
+// 2.3e-17 relative with the coefficients rounded to double, so the result is
+// within about 1 ulp of the correctly rounded value.  Arguments below -746
+// give 0 (e^-746 is below half the smallest subnormal).
+__device__ __forceinline__ double exp_minimax(double x) {
     const double LOG2E = 1.4426950408889634;
     const double LN2_HI = 6.93147180369123816490e-01;
     const double LN2_LO = 1.90821492927058770002e-10;
-    if (x < -746.0) x = -746.0;
+    x = fmax(x, -746.0);
     const double k = rint(x * LOG2E);
     double r = fma(-k, LN2_HI, x);
     r = fma(-k, LN2_LO, r);
-    double p = fma_sconst(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
-    p = fma_sconst(p, r, 1.0 / 39916800.0);
-    p = fma_sconst(p, r, 1.0 / 3628800.0);
-    p = fma_sconst(p, r, 1.0 / 362880.0);
-    p = fma_sconst(p, r, 1.0 / 40320.0);
-    p = fma_sconst(p, r, 1.0 / 5040.0);
-    p = fma_sconst(p, r, 1.0 / 720.0);
-    p = fma_sconst(p, r, 1.0 / 120.0);
-    p = fma_sconst(p, r, 1.0 / 24.0);
-    p = fma_sconst(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
+    double p = fma_sconst(2.4994304015873615e-08, r, 2.7632293297646363e-07);
+    p = fma_sconst(p, r, 2.7557622533560847e-06);
+    p = fma_sconst(p, r, 2.4801486521375564e-05);
+    p = fma_sconst(p, r, 1.9841269432676259e-04);
+    p = fma_sconst(p, r, 1.388888895122404e-03);
+    p = fma_sconst(p, r, 8.333333333559272e-03);
+    p = fma_sconst(p, r, 4.166666666649277e-02);
+    p = fma_sconst(p, r, 1.666666666666617e-01);
+    p = fma_sconst(p, r, 5.000000000000018e-01);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
     return ldexp(p, (int)k);
@@ -153,7 +153,7 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
 
 // g(r) as above from |d| and 1/|d| with products in place of the
 // reference's divisions (norm/rm, 1/r, g/norm), the dispersion terms
-// factored as polynomials in ix^2, and exp_taylor in place of the library
+// factored as polynomials in ix^2, and exp_minimax in place of the library
 // exp: a few ulp apart.
 template <int KIND>
 __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_norm, double kpair, double strength) {
@@ -165,7 +165,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
     } else if constexpr (KIND == POT_AZIZ) {
         constexpr double inv_rm = 1.0 / Aziz::rm;
         const double r = norm * inv_rm;
-        const double T1 = -Aziz::A * Aziz::alpha * exp_taylor(-Aziz::alpha * r);
+        const double T1 = -Aziz::A * Aziz::alpha * exp_minimax(-Aziz::alpha * r);
         double g;
         if (r > HARD_CORE_EPS && r < 0.01) {
             g = T1 * (Aziz::epsilon / Aziz::rm);
@@ -180,7 +180,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
             if (r < Aziz::D) {
                 const double pb = fma_sconst(fma_sconst(Aziz::C10, ix2, Aziz::C8), ix2, Aziz::C6);
                 const double t = Aziz::D * ix - 1.0;
-                const double ex = exp_taylor(-t * t);
+                const double ex = exp_minimax(-t * t);
                 attr = (attr - (2.0 * Aziz::D) * ix2 * t * pb) * ex;
             }
             const double T2T3 = ix6 * attr;

@@ -61,12 +61,12 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
                 constexpr int IMG = decltype(img)::value;
                 for (int j = jlo; j < jhi; ++j) {
                     if (j == i) continue;
-                    double xj[D], fij[D];
+                    double xj[D], d[D], g;
 #pragma unroll
                     for (int a = 0; a < D; ++a) xj[a] = sx[a * N + j];
-                    if (pair_force<D, KIND, IMG>(xi, xj, pp, fij)) {
+                    if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
 #pragma unroll
-                        for (int a = 0; a < D; ++a) acc[a] += fij[a];
+                        for (int a = 0; a < D; ++a) acc[a] = fma(-g, d[a], acc[a]);
                     }
                 }
             };

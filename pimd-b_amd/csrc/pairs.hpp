@@ -61,14 +61,24 @@ __device__ __forceinline__ double min_image_pair(double dx, const PairParams& pp
 
 __host__ __device__ inline int pair_image(const PairParams& pp) { return !pp.pbc ? 0 : (pp.guard ? 2 : 1); }
 
-// 1/sqrt(r2) to double precision: hardware estimate + two Newton steps
-// (each squares the relative error).
+// 1/sqrt(r2) to double precision: the hardware estimate y0 and one
+// third-order correction.  With e = 1 - r2 y0^2, 1/sqrt(r2) = y0 (1-e)^(-1/2)
+// = y0 (1 + e/2 + 3e^2/8 + 5e^3/16 + ...); the dropped term is 5e^3/16, far
+// below an ulp for the estimate's |e| ~ 2^-23.
 __device__ __forceinline__ double rsqrt_refined(double r2) {
-    double y = __builtin_amdgcn_rsq(r2);
-    const double h = 0.5 * r2;
-    y = y * (1.5 - h * y * y);
-    y = y * (1.5 - h * y * y);
-    return y;
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double e = fma(-r2, y0 * y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
+// Squared norm with fused multiply-adds (the pair loop; the exchange keeps
+// the reference's unfused sum, norm2).
+template <int D>
+__device__ __forceinline__ double norm2_fma(const double (&d)[D]) {
+    double s = d[0] * d[0];
+#pragma unroll
+    for (int a = 1; a < D; ++a) s = fma(d[a], d[a], s);
+    return s;
 }
 
 // The reference's strict test |d| < cutoff (simulation.cpp:444) for the rare
@@ -77,22 +87,20 @@ __device__ __forceinline__ double rsqrt_refined(double r2) {
 // the compiler does not evaluate the square root for every pair.
 static __device__ __noinline__ bool within_cutoff_exact(double r2, double cutoff) { return sqrt(r2) < cutoff; }
 
-// Force on i due to j, -gradV(r_i - r_j), if the pair is inside the cutoff
-// (distance < cutoff, or cutoff < 0); false otherwise.
+// gradV(r_i - r_j) = g d for the pair (i, j): d = r_i - r_j (min-imaged) and
+// g, if the pair is inside the cutoff (distance < cutoff, or cutoff < 0);
+// false otherwise.  The force on i is -g d, on j +g d.
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ bool pair_force(const double (&xi)[D], const double (&xj)[D], const PairParams& pp,
-                                           double (&f)[D]) {
-    double d[D];
+__device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&xj)[D], const PairParams& pp,
+                                          double (&d)[D], double& g) {
 #pragma unroll
     for (int a = 0; a < D; ++a) d[a] = min_image_pair<IMG>(xi[a] - xj[a], pp);
-    const double r2 = norm2<D>(d);
+    const double r2 = norm2_fma<D>(d);
     bool in = pp.cutoff < 0.0 || r2 < pp.cut2_lo;
     if (!in && r2 < pp.cut2_hi) in = within_cutoff_exact(r2, pp.cutoff);
     if (!in) return false;
     const double inv = rsqrt_refined(r2);
-    const double g = pair_grad_factor_fast<KIND>(r2 * inv, inv, pp.int_k, pp.strength);
-#pragma unroll
-    for (int a = 0; a < D; ++a) f[a] = -(g * d[a]);
+    g = pair_grad_factor_fast<KIND>(r2 * inv, inv, pp.int_k, pp.strength);
     return true;
 }
 
@@ -134,12 +142,12 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
             }
             for (int s = s0; s < s1; ++s) {
                 if (row_ok && jc < N && (!diag || s < 32 || lane < 32)) {
-                    double f[D];
-                    if (pair_force<D, KIND, IMG>(xi, xj, pp, f)) {
+                    double d[D], g;
+                    if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
-                            racc[a] += f[a];
-                            cacc[a] -= f[a];
+                            racc[a] = fma(-g, d[a], racc[a]);
+                            cacc[a] = fma(g, d[a], cacc[a]);
                         }
                     }
                 }

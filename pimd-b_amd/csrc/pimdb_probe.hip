@@ -1,9 +1,11 @@
 // pimdb_probe.hip -- micro-benchmarks for the roofline: FP64 FMA rate, FP64
-// exp throughput of the device math library, and HBM copy bandwidth.  The
-// copy kernel also serves as the known-size reference kernel when hardware
-// byte counters are scaled.
+// exp throughput (exp_minimax, the exp of the pair kernels), and HBM copy
+// bandwidth.  The copy kernel also serves as the known-size reference kernel
+// when hardware byte counters are scaled.  pimdb_math_probe evaluates the
+// device elementary functions of the pair kernels on caller data.
 #include <hip/hip_runtime.h>
 
+#include "pairs.hpp"
 #include "pimdb_gpu.h"
 
 namespace {
@@ -30,12 +32,19 @@ __global__ void __launch_bounds__(256) k_probe_exp(double* out, int iters) {
     for (int it = 0; it < iters; ++it) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const double e = exp(v[k]);
+            const double e = pimdb::exp_minimax(v[k]);
             s += e;
             v[k] -= 1e-9;
         }
     }
     if (s == 12345.678) out[0] = s;
+}
+
+__global__ void __launch_bounds__(256) k_probe_math(int fn, const double* __restrict__ x, double* __restrict__ y,
+                                                    long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = fn == 0 ? pimdb::exp_minimax(x[i]) : pimdb::rsqrt_refined(x[i]);
 }
 
 __global__ void __launch_bounds__(256) k_probe_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
@@ -100,5 +109,25 @@ extern "C" int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, doub
     (void)hipFree(sink);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return rc;
+}
+
+extern "C" int pimdb_math_probe(int fn, const double* x, double* y, long long n) {
+    if ((fn != 0 && fn != 1) || n < 0 || (n > 0 && (!x || !y))) return PIMDB_E_ARG;
+    if (n == 0) return PIMDB_OK;
+    double *dx = nullptr, *dy = nullptr;
+    int rc = PIMDB_OK;
+    const size_t bytes = sizeof(double) * (size_t)n;
+    if (hipMalloc((void**)&dx, bytes) != hipSuccess || hipMalloc((void**)&dy, bytes) != hipSuccess ||
+        hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = PIMDB_E_HIP;
+    } else {
+        const long long blocks = (n + 255) / 256;
+        hipLaunchKernelGGL(k_probe_math, dim3((unsigned)blocks), dim3(256), 0, 0, fn, dx, dy, n);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(y, dy, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = PIMDB_E_HIP;
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
     return rc;
 }

@@ -90,6 +90,7 @@ def lib():
         "pimdb_graph_capture": (i, [vp]),
         "pimdb_time_kernels": (i, [vp, i, _DP, _DP, _DP]),
         "pimdb_peak_probe": (i, [_DP, _DP, _DP, C.c_longlong]),
+        "pimdb_math_probe": (i, [i, _DP, _DP, C.c_longlong]),
         "pimdb_xplan_create": (i, [i, i, d, d, i, d, C.POINTER(vp)]),
         "pimdb_xplan_destroy": (None, [vp]),
         "pimdb_xplan_forward": (i, [vp, vp, vp, vp, vp, vp, vp]),
@@ -291,6 +292,17 @@ def peak_probe(copy_bytes=1 << 30):
     if rc != OK:
         _raise(rc, "pimdb_peak_probe failed")
     return tuple(float(a[0]) for a in v)
+
+
+def math_probe(fn, x):
+    """The pair kernels' device e^x (fn "exp") or 1/sqrt(x) (fn "rsqrt")
+    evaluated on the GPU at the values x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    rc = lib().pimdb_math_probe({"exp": 0, "rsqrt": 1}[fn], _p(x), _p(y), int(x.size))
+    if rc != OK:
+        _raise(rc, "pimdb_math_probe failed")
+    return y
 
 
 class XPlan:
