@@ -105,6 +105,26 @@ typedef const __attribute__((address_space(4))) double* const_dptr;
 // two modes of a Box-Muller pair land on the same lane.
 constexpr int NM_BLOCK = 4;
 
+// Normal-mode transforms on the matrix cores (v_mfma_f64_16x16x4f64) for
+// P <= NM_MFMA_MAXP.  Lane layout of that instruction (lane l, item v):
+// A[l%16][l/16], B[l/16][l%16], D[l/16 + 4v][l%16].
+constexpr int NM_MFMA_MAXP = 64;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int nm_pad16(int P) { return (P + 15) & ~15; }
+// Column blocks of 16 that one round of the transform covers: one wave per
+// (16-mode block, column block).
+__host__ __device__ inline int nm_mfma_groups(int P, int nwaves) {
+    const int mb = nm_pad16(P) / 16;
+    return nwaves >= mb ? nwaves / mb : 1;
+}
+// LDS doubles of the matrix-core path: both matrices padded to Pp x Pp and
+// two column tiles [groups][Pp][16].
+__host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
+    const size_t Pp = (size_t)nm_pad16(P);
+    return 2 * Pp * Pp + 2 * (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
+}
+
 // p[b][a][*] -= cm[a] for every bead and axis of a replica.
 template <int D>
 __device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double (&cm)[D]) {
@@ -136,7 +156,101 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     double csum[D];   // this thread's share of the momentum sums per axis
 #pragma unroll
     for (int a = 0; a < D; ++a) csum[a] = 0.0;
-    if (r.thermostat != 1 || !r.nm) {
+    if (r.thermostat == 1 && r.nm && P <= NM_MFMA_MAXP) {
+        // Matrix-core form.  LDS: Cl[j][k] = C[k][j] and CTl[k][b] = CT[b][k]
+        // (Pp x Pp, zero padded), pc[g][j][c] the staged columns (after the
+        // half kick) and pn[g][k][c] the thermostatted modes, g = column block
+        // of the round.  Wave (mb, g) computes 16 modes of 16 columns, then 16
+        // beads of them.  Its A rows are permuted so that lane q holds the
+        // four consecutive modes 16mb + 4q .. 4q+3: modes 2m and 2m+1 of a
+        // Box-Muller pair are on one lane.
+        const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+        const int wave = uni((int)(threadIdx.x / WAVE));
+        const int Pp = nm_pad16(P), MB = Pp / 16, KK = Pp / 4;
+        const int G = nm_mfma_groups(P, nw), ntask = MB * G;
+        const int NCB = (ncols + 15) / 16;
+        double* Cl = lds;
+        double* CTl = Cl + (size_t)Pp * Pp;
+        double* pc = CTl + (size_t)Pp * Pp;
+        double* pn = pc + (size_t)G * Pp * 16;
+        const double* Ct = uni(bp.Ct) + r.off_C;
+        const double* CTt = uni(bp.CTt) + r.off_C;
+        for (int e = threadIdx.x; e < Pp * Pp; e += blockDim.x) {
+            const int a = e / Pp, b = e % Pp;
+            const bool in = a < P && b < P;
+            Cl[e] = in ? Ct[a * P + b] : 0.0;
+            CTl[e] = in ? CTt[a * P + b] : 0.0;
+        }
+        const int q = lane / 16, c16 = lane % 16;
+        const int arow = 4 * (c16 % 4) + c16 / 4;   // permuted A row of this lane
+        const int width = G * 16;
+        for (int cb0 = 0; cb0 < NCB; cb0 += G) {
+            const int c0 = cb0 * 16;
+            for (int e = threadIdx.x; e < Pp * width; e += blockDim.x) {
+                const int j = e / width, gc = e % width, c = c0 + gc;
+                double v = 0.0;
+                if (j < P && c < ncols) {
+                    const size_t g = (size_t)j * ncols + c;
+                    v = p[g];
+                    if (kick) v += half_dt * f[g];
+                }
+                pc[((size_t)(gc / 16) * Pp + j) * 16 + gc % 16] = v;
+            }
+            __syncthreads();
+            for (int task = wave; task < ntask; task += nw) {
+                const int mb = task % MB, g = task / MB;
+                if (cb0 + g >= NCB) continue;
+                const double* B = pc + (size_t)g * Pp * 16;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+                for (int kk = 0; kk < KK; ++kk) {
+                    const int k = 4 * kk + q;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Cl[(size_t)k * Pp + mb * 16 + arow],
+                                                                B[k * 16 + c16], acc, 0, 0, 0);
+                }
+                // acc[v] = mode 16mb + 4q + v of column c0 + 16g + c16
+                const int col = c0 + g * 16 + c16;
+                double* out = pn + (size_t)g * Pp * 16;
+#pragma unroll
+                for (int h = 0; h < 4; h += 2) {
+                    const int k = mb * 16 + 4 * q + h;
+                    double g0 = 0.0, g1 = 0.0;
+                    if (k < P && col < ncols)
+                        gauss2((uint32_t)((size_t)(k >> 1) * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                    out[k * 16 + c16] = k < P ? r.c1 * acc[h] + r.c2 * g0 : 0.0;
+                    out[(k + 1) * 16 + c16] = k + 1 < P ? r.c1 * acc[h + 1] + r.c2 * g1 : 0.0;
+                }
+            }
+            __syncthreads();
+            for (int task = wave; task < ntask; task += nw) {
+                const int bb = task % MB, g = task / MB;
+                if (cb0 + g >= NCB) continue;
+                const double* B = pn + (size_t)g * Pp * 16;
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+                for (int kk = 0; kk < KK; ++kk) {
+                    const int k = 4 * kk + q;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CTl[(size_t)k * Pp + bb * 16 + c16],
+                                                                B[k * 16 + c16], acc, 0, 0, 0);
+                }
+                // acc[v] = bead 16bb + q + 4v of column c0 + 16g + c16
+                const int col = c0 + g * 16 + c16;
+                if (col < ncols) {
+                    const int ax = col / N;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int b = bb * 16 + q + 4 * v;
+                        if (b < P) {
+                            p[(size_t)b * ncols + col] = acc[v];
+#pragma unroll
+                            for (int a = 0; a < D; ++a)
+                                if (a == ax) csum[a] += acc[v];
+                        }
+                    }
+                }
+            }
+            // the next round's staging waits at its barrier for these reads
+        }
+        __syncthreads();
+    } else if (r.thermostat != 1 || !r.nm) {
         // beads 2m and 2m+1 of a column share one Box-Muller pair
         const int half = (P + 1) / 2;
         for (int e2 = threadIdx.x; e2 < half * ncols; e2 += blockDim.x) {
@@ -334,8 +448,11 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     __syncthreads();
 }
 
-// LDS doubles of the integrator's normal-mode column tiles.
-__host__ __device__ inline size_t batch_nm_lds_doubles(int P) { return 2 * (size_t)P * WAVE; }
+// LDS doubles of the integrator's normal-mode transforms (matrix-core form
+// for P <= NM_MFMA_MAXP, column tiles of 64 otherwise) with nwaves waves.
+__host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
+    return P <= NM_MFMA_MAXP ? nm_mfma_lds_doubles(P, nwaves) : 2 * (size_t)P * WAVE;
+}
 
 // Exchange of one replica: cycle energies, both recursions, exterior forces
 // added into f of beads 1 and P.  Kept out of line (its registers would

@@ -63,6 +63,25 @@ def trap_replicas(n):
     return out
 
 
+def nm_replicas(n):
+    """Normal-mode Langevin replicas with bead counts that are not multiples
+    of 16 and above 64 (the two forms of the batched normal-mode transform),
+    uneven N, harmonic trap and pairs."""
+    rng = np.random.default_rng(31)
+    out = []
+    for r in range(n):
+        N, P = int([8, 20, 33, 12, 17, 9][r % 6]), int([5, 24, 40, 72, 96, 1][r % 6])
+        kw = dict(natoms=N, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+                  gamma=1.0 / 500.0, bosonic=int(r % 2 == 0), pbc=0, size=30.0, ext_kind="harmonic",
+                  ext_omega=2.0 * MEV, int_kind="harmonic", int_omega=1.0 * MEV, cutoff=-1.0,
+                  propagator="cartesian", thermostat="langevin", nmthermostat=1, fixcom=int(r % 3 != 1),
+                  seed=900 + r)
+        x = rng.uniform(-5.0, 5.0, size=(P, N, 3))
+        p = rng.normal(0, 0.5, size=(P, N, 3))
+        out.append((kw, x, p))
+    return out
+
+
 @pytest.mark.parametrize("which", ["dipole", "aziz", "traps"])
 def test_batch_forces_match_oracle(which):
     reps = {"dipole": lambda: dipole_replicas(12, seed=3), "aziz": lambda: aziz_replicas(6),
@@ -78,9 +97,10 @@ def test_batch_forces_match_oracle(which):
     b.close()
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz"])
+@pytest.mark.parametrize("which", ["dipole", "aziz", "nm"])
 def test_batch_follows_single_system(which):
-    reps = dipole_replicas(6, seed=5) if which == "dipole" else aziz_replicas(4)
+    reps = {"dipole": lambda: dipole_replicas(6, seed=5), "aziz": lambda: aziz_replicas(4),
+            "nm": lambda: nm_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     ctxs = []
     for r, (kw, x, p) in enumerate(reps):
