@@ -235,8 +235,9 @@ def main():
     t_roof = fl / (PEAK_FP64_TFLOPS * 1e12) + xp / (exp_g * 1e9)
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic_from_profile(),
-                    compute="FP64 on the VALU: MI355X's FP64 vector peak equals its FP64 matrix peak "
-                            "(78.6 TFLOP/s); the kernel issues no MFMA instruction",
+                    compute="FP64: pairs, exchange and springs on the VALU, the normal-mode transforms "
+                            "(19% of the flops) on the matrix cores (v_mfma_f64_16x16x4f64); MI355X's FP64 "
+                            "vector peak equals its FP64 matrix peak (78.6 TFLOP/s)",
                     kernel="k_batch_step<3,AZIZ,STEP> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
                     timing="hipEvents on the batch stream around the timed steps",
                     flops_per_launch=fl, exps_per_launch=xp, frac_with_exps=t_roof / (ms_launch * 1e-3))

@@ -125,13 +125,37 @@ __host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
     return 2 * Pp * Pp + 2 * (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
 }
 
+// Element loops over a replica's [bead][axis][particle] arrays take
+// STREAM_UNROLL elements per thread per pass and issue all their loads before
+// any store, so each thread keeps that many loads in flight.
+constexpr int STREAM_UNROLL = 4;
+
+// LDS pointer type: accesses through it are ds_ instructions that the
+// compiler knows cannot alias global memory.
+typedef __attribute__((address_space(3))) double lds_double;
+
+// Axis of element e of a [bead][axis][particle] array.
+template <int D>
+__device__ __forceinline__ int elem_axis(int e, int N) {
+    return D == 1 ? 0 : (e / N) % D;
+}
+
 // p[b][a][*] -= cm[a] for every bead and axis of a replica.
 template <int D>
 __device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double (&cm)[D]) {
-    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE, wave = threadIdx.x / WAVE;
-    for (int row = wave; row < P * D; row += nw) {
-        const double c = cm[row % D];
-        for (int i = lane; i < N; i += WAVE) p[(size_t)row * N + i] -= c;
+    const int n = P * D * N, nthr = blockDim.x;
+    for (int base = threadIdx.x; base < n; base += STREAM_UNROLL * nthr) {
+        double pv[STREAM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < STREAM_UNROLL; ++u) {
+            const int e = base + u * nthr;
+            if (e < n) pv[u] = p[e];
+        }
+#pragma unroll
+        for (int u = 0; u < STREAM_UNROLL; ++u) {
+            const int e = base + u * nthr;
+            if (e < n) p[e] = pv[u] - cm[elem_axis<D>(e, N)];
+        }
     }
 }
 
@@ -169,10 +193,10 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         const int Pp = nm_pad16(P), MB = Pp / 16, KK = Pp / 4;
         const int G = nm_mfma_groups(P, nw), ntask = MB * G;
         const int NCB = (ncols + 15) / 16;
-        double* Cl = lds;
-        double* CTl = Cl + (size_t)Pp * Pp;
-        double* pc = CTl + (size_t)Pp * Pp;
-        double* pn = pc + (size_t)G * Pp * 16;
+        lds_double* Cl = (lds_double*)lds;
+        lds_double* CTl = Cl + (size_t)Pp * Pp;
+        lds_double* pc = CTl + (size_t)Pp * Pp;
+        lds_double* pn = pc + (size_t)G * Pp * 16;
         const double* Ct = uni(bp.Ct) + r.off_C;
         const double* CTt = uni(bp.CTt) + r.off_C;
         for (int e = threadIdx.x; e < Pp * Pp; e += blockDim.x) {
@@ -186,21 +210,35 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         const int width = G * 16;
         for (int cb0 = 0; cb0 < NCB; cb0 += G) {
             const int c0 = cb0 * 16;
-            for (int e = threadIdx.x; e < Pp * width; e += blockDim.x) {
-                const int j = e / width, gc = e % width, c = c0 + gc;
-                double v = 0.0;
-                if (j < P && c < ncols) {
-                    const size_t g = (size_t)j * ncols + c;
-                    v = p[g];
-                    if (kick) v += half_dt * f[g];
+            const int nst = Pp * width, nthr = blockDim.x;
+            for (int base = threadIdx.x; base < nst; base += STREAM_UNROLL * nthr) {
+                double pv[STREAM_UNROLL], fv[STREAM_UNROLL];
+#pragma unroll
+                for (int u = 0; u < STREAM_UNROLL; ++u) {
+                    const int e = base + u * nthr;
+                    const int j = e / width, c = c0 + e % width;
+                    pv[u] = 0.0;
+                    fv[u] = 0.0;
+                    if (e < nst && j < P && c < ncols) {
+                        const size_t g = (size_t)j * ncols + c;
+                        pv[u] = p[g];
+                        if (kick) fv[u] = f[g];
+                    }
                 }
-                pc[((size_t)(gc / 16) * Pp + j) * 16 + gc % 16] = v;
+#pragma unroll
+                for (int u = 0; u < STREAM_UNROLL; ++u) {
+                    const int e = base + u * nthr;
+                    if (e < nst) {
+                        const int j = e / width, gc = e % width;
+                        pc[((size_t)(gc / 16) * Pp + j) * 16 + gc % 16] = kick ? pv[u] + half_dt * fv[u] : pv[u];
+                    }
+                }
             }
             __syncthreads();
             for (int task = wave; task < ntask; task += nw) {
                 const int mb = task % MB, g = task / MB;
                 if (cb0 + g >= NCB) continue;
-                const double* B = pc + (size_t)g * Pp * 16;
+                const lds_double* B = pc + (size_t)g * Pp * 16;
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
                 for (int kk = 0; kk < KK; ++kk) {
                     const int k = 4 * kk + q;
@@ -209,7 +247,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                 }
                 // acc[v] = mode 16mb + 4q + v of column c0 + 16g + c16
                 const int col = c0 + g * 16 + c16;
-                double* out = pn + (size_t)g * Pp * 16;
+                lds_double* out = pn + (size_t)g * Pp * 16;
 #pragma unroll
                 for (int h = 0; h < 4; h += 2) {
                     const int k = mb * 16 + 4 * q + h;
@@ -224,7 +262,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             for (int task = wave; task < ntask; task += nw) {
                 const int bb = task % MB, g = task / MB;
                 if (cb0 + g >= NCB) continue;
-                const double* B = pn + (size_t)g * Pp * 16;
+                const lds_double* B = pn + (size_t)g * Pp * 16;
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
                 for (int kk = 0; kk < KK; ++kk) {
                     const int k = 4 * kk + q;
@@ -252,24 +290,44 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         __syncthreads();
     } else if (r.thermostat != 1 || !r.nm) {
         // beads 2m and 2m+1 of a column share one Box-Muller pair
-        const int half = (P + 1) / 2;
-        for (int e2 = threadIdx.x; e2 < half * ncols; e2 += blockDim.x) {
-            const int m = e2 / ncols, col = e2 % ncols;
-            const int ax = col / N;
-            double g[2] = {0.0, 0.0};
-            if (r.thermostat == 1) gauss2((uint32_t)e2, step, ostep, r.rid, r.seed, g[0], g[1]);
+        const int half = (P + 1) / 2, n2 = half * ncols, nthr = blockDim.x;
+        constexpr int U = STREAM_UNROLL / 2;
+        for (int base = threadIdx.x; base < n2; base += U * nthr) {
+            double pv[U][2], fv[U][2];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int b = 2 * m + h;
-                if (b < P) {
-                    const size_t e = (size_t)b * ncols + col;
-                    double v = p[e];
-                    if (kick) v += half_dt * f[e];
-                    if (r.thermostat == 1) v = r.c1 * v + r.c2 * g[h];
-                    p[e] = v;
+            for (int u = 0; u < U; ++u) {
+                const int e2 = base + u * nthr, m = e2 / ncols, col = e2 % ncols;
 #pragma unroll
-                    for (int a = 0; a < D; ++a)
-                        if (a == ax) csum[a] += v;
+                for (int h = 0; h < 2; ++h) {
+                    const int b = 2 * m + h;
+                    pv[u][h] = 0.0;
+                    fv[u][h] = 0.0;
+                    if (e2 < n2 && b < P) {
+                        const size_t e = (size_t)b * ncols + col;
+                        pv[u][h] = p[e];
+                        if (kick) fv[u][h] = f[e];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e2 = base + u * nthr, m = e2 / ncols, col = e2 % ncols;
+                if (e2 >= n2) continue;
+                const int ax = col / N;
+                double g[2] = {0.0, 0.0};
+                if (r.thermostat == 1) gauss2((uint32_t)e2, step, ostep, r.rid, r.seed, g[0], g[1]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int b = 2 * m + h;
+                    if (b < P) {
+                        double v = pv[u][h];
+                        if (kick) v += half_dt * fv[u][h];
+                        if (r.thermostat == 1) v = r.c1 * v + r.c2 * g[h];
+                        p[(size_t)b * ncols + col] = v;
+#pragma unroll
+                        for (int a = 0; a < D; ++a)
+                            if (a == ax) csum[a] += v;
+                    }
                 }
             }
         }
@@ -379,16 +437,28 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
     double* p = uni(bp.p) + r.off_x;
     const double* f = uni(bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
-    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE, wave = threadIdx.x / WAVE;
-    for (int row = wave; row < P * D; row += nw) {
-        const double c = cm[row % D];
-        for (int i = lane; i < N; i += WAVE) {
-            const size_t e = (size_t)row * N + i;
-            double pv = p[e];
-            if (r.fixcom) pv -= c;
-            pv = pv + half_dt * f[e];
-            p[e] = pv;
-            x[e] += r.dt * pv / r.mass;
+    const int n = P * D * N, nthr = blockDim.x;
+    for (int base = threadIdx.x; base < n; base += STREAM_UNROLL * nthr) {
+        double pv[STREAM_UNROLL], fv[STREAM_UNROLL], xv[STREAM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < STREAM_UNROLL; ++u) {
+            const int e = base + u * nthr;
+            if (e < n) {
+                pv[u] = p[e];
+                fv[u] = f[e];
+                xv[u] = x[e];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < STREAM_UNROLL; ++u) {
+            const int e = base + u * nthr;
+            if (e < n) {
+                double v = pv[u];
+                if (r.fixcom) v -= cm[elem_axis<D>(e, N)];
+                v = v + half_dt * fv[u];
+                p[e] = v;
+                x[e] = xv[u] + r.dt * v / r.mass;
+            }
         }
     }
     __syncthreads();
