@@ -57,9 +57,6 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
 #ifndef BATCH_REC_UNROLL
 #define BATCH_REC_UNROLL 4
 #endif
-#ifndef BATCH_EXT_UNROLL
-#define BATCH_EXT_UNROLL 1
-#endif
 
 // Wave-uniform copies (v_readfirstlane) of values that reach a non-inlined
 // function in vector registers.
@@ -541,8 +538,6 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     int* status = uni(status_);
     double* lds = uni(lds_);
     const size_t bs = (size_t)D * N;
-    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
-    const int wave = uni((int)(threadIdx.x / WAVE));
     // beads 1 and P and the link prefix sums in LDS; cycle energies are
     // recomputed from them wherever they are needed
     const RecShared sh = rec_carve(lds, N, D);
@@ -552,20 +547,7 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     __threadfence_block();
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
-    for (int task = wave; task < 2 * N; task += nw) {
-        const int which = task / N, l = task % N;
-        double F[D];
-        exterior_force<D, BATCH_EXT_UNROLL>(which, l, N, lane, bt, k, r, V, Vb, x2, xPm1, false, F);
-        if (lane == 0) {
-            const size_t b = which == 0 ? 0 : (size_t)(P - 1);
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const size_t idx = (b * D + a) * N + l;
-                f[idx] = F[a] + f[idx];
-            }
-        }
-    }
-    __syncthreads();
+    wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_shared_doubles(N, D));
 }
 
 // Work of one launch: a full MD step, or parts of it for timing.

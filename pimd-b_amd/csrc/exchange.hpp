@@ -503,6 +503,140 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
     }
 }
 
+// The exterior spring forces of beads 1 and P for one ring on a whole
+// workgroup (the batched form), the same sums as exterior_force arranged
+// lane-per-particle.  Thread t takes particle l = t mod N and the t/N-th of
+// `splits` equal ranges of j.  Every j gives particle l exactly one term of
+// the main loop: the bead-1 term when j >= l-1 (P[j][l], the link at
+// j = l-1), the bead-P term when j < l-1 (P[l][j]); the bead-P terms at
+// j = l-1, l and the bead-P link at j = l+1 are added by the first range.
+// Lanes of a wave share j, so the ring, V, Vb and 1/(i+1) are LDS broadcast
+// reads, and each lane's load is the same length.  Partial sums go through
+// LDS and are added in range order; f[bead][a][l] += k (S + interior link).
+// Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x).
+__host__ __device__ inline int ext_splits(int N, int nthreads) { return N < nthreads ? nthreads / N : 1; }
+__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
+    const int sp = ext_splits(N, nthreads);
+    return 3 * (size_t)N + 2 + (sp > 1 ? (size_t)sp * 2 * D * N : 0);
+}
+
+template <int D>
+__device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, const Ring& r,
+                                                   const double* __restrict__ V, const double* __restrict__ Vb,
+                                                   const double* __restrict__ x2, const double* __restrict__ xPm1,
+                                                   double* __restrict__ f1, double* __restrict__ fP, double* scratch) {
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    double* Vl = scratch;             // [N+1]
+    double* Vbl = Vl + N + 1;         // [N+1]
+    double* inv1 = Vbl + N + 1;       // [N]   1/(i+1)
+    double* part = inv1 + N;          // [splits][2][D][N]
+    for (int i = tid; i <= N; i += nthr) {
+        Vl[i] = V[i];
+        Vbl[i] = Vb[i];
+        if (i < N) inv1[i] = 1.0 / (i + 1);
+    }
+    __syncthreads();
+    const int splits = ext_splits(N, nthr);
+    const int jlen = (N + splits - 1) / splits;
+    const double VN = Vl[N];
+    const int lanes_used = splits > 1 ? splits * N : nthr;
+    for (int base = 0; base < N; base += (splits > 1 ? N : nthr)) {
+        const int l = splits > 1 ? tid % N : base + tid;
+        const int sp = splits > 1 ? tid / N : 0;
+        const bool act = tid < lanes_used && l < N && sp < splits;
+        double S0[D], S1[D], x1l[D], xPl[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            S0[a] = 0.0;
+            S1[a] = 0.0;
+            x1l[a] = act ? r.x1[a * N + l] : 0.0;
+            xPl[a] = act ? r.xP[a * N + l] : 0.0;
+        }
+        if (act) {
+            const double shl = r.Shi[l], sll = r.Slo[l];
+            const double Vl_l = Vl[l], Vb_l1 = Vbl[l + 1], w_l = inv1[l];
+            // bead-1 link (j = l-1): 1 - exp(-bt (V[l] + Vb[l] - V[N]))
+            const double prlink = l >= 1 ? 1.0 - exp_minimax(-bt * (Vl_l + Vbl[l] - VN)) : 0.0;
+            const int j0 = sp * jlen, j1 = min(N, j0 + jlen);
+            for (int j = j0; j < j1; ++j) {
+                const bool selP = j < l - 1;   // bead-P term P[l][j]; else bead-1 term P[j][l]
+                double d[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const double o = selP ? r.x1[a * N + j] : r.xP[a * N + j];
+                    const double sf = selP ? xPl[a] : x1l[a];
+                    d[a] = ring_image(o - sf, r);
+                }
+                double pr;
+                if (j == l - 1) {
+                    pr = prlink;
+                } else {
+                    const double shj = r.Shi[j], slj = r.Slo[j];
+                    // ecyc(u, v): bead 1 (u, v) = (l, j), bead P (u, v) = (j, l)
+                    const double e = selP ? ring_ecyc<D>(r, d, shj, slj, shl, sll)
+                                          : ring_ecyc<D>(r, d, shl, sll, shj, slj);
+                    const double A = selP ? Vl[j] : Vl_l;
+                    const double B = selP ? Vb_l1 : Vbl[j + 1];
+                    const double w = selP ? w_l : inv1[j];
+                    pr = w * exp_minimax(-bt * (A + e + B - VN));
+                }
+                const double p0 = selP ? 0.0 : pr, p1 = selP ? pr : 0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    S0[a] = fma(p0, d[a], S0[a]);
+                    S1[a] = fma(p1, d[a], S1[a]);
+                }
+            }
+            if (sp == 0) {
+                // bead-P terms j = l-1, l (cycles through j..l) and the link j = l+1
+                for (int j = max(0, l - 1); j <= min(l + 1, N - 1); ++j) {
+                    double d[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) d[a] = ring_image(r.x1[a * N + j] - xPl[a], r);
+                    double pr;
+                    if (j == l + 1) {
+                        pr = 1.0 - exp_minimax(-bt * (Vl[l + 1] + Vb_l1 - VN));
+                    } else {
+                        const double e = ring_ecyc<D>(r, d, r.Shi[j], r.Slo[j], shl, sll);
+                        pr = w_l * exp_minimax(-bt * (Vl[j] + e + Vb_l1 - VN));
+                    }
+#pragma unroll
+                    for (int a = 0; a < D; ++a) S1[a] = fma(pr, d[a], S1[a]);
+                }
+            }
+        }
+        if (splits > 1) {
+            if (act) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    part[(((size_t)sp * 2 + 0) * D + a) * N + l] = S0[a];
+                    part[(((size_t)sp * 2 + 1) * D + a) * N + l] = S1[a];
+                }
+            }
+            __syncthreads();
+            if (act && sp == 0) {
+                for (int q = 1; q < splits; ++q) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        S0[a] += part[(((size_t)q * 2 + 0) * D + a) * N + l];
+                        S1[a] += part[(((size_t)q * 2 + 1) * D + a) * N + l];
+                    }
+                }
+            }
+        }
+        if (act && sp == 0) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const double s0 = S0[a] + ring_image(x2[a * N + l] - x1l[a], r);
+                const double s1 = S1[a] + ring_image(xPm1[a * N + l] - xPl[a], r);
+                f1[a * N + l] = s0 * k + f1[a * N + l];
+                fP[a * N + l] = s1 * k + fP[a * N + l];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Adds the exterior spring forces into f for beads 1 and P:
 // f = F_exchange + f_phys (simulation.cpp:363-373: spring + physical).
 // Shi/Slo: the ring's prefix sums from k_exchange_recursions.
