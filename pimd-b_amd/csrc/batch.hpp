@@ -22,6 +22,7 @@ namespace pimdb {
 struct RepDesc {
     int N, P, bosonic, pbc;
     int thermostat, nm, fixcom;
+    int xdual;                  // exchange recursions side by side (LDS permitting)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
     double int_k, strength, cutoff;
@@ -81,7 +82,7 @@ __device__ __forceinline__ T* uni(T* p) {
 __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
-    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom);
+    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.xdual = uni(r.xdual);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
     u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
@@ -527,8 +528,9 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
 // on entry.
 template <int D>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
-                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_) {
+                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_, int dual_) {
     const int N = uni(N_), P = uni(P_);
+    const bool dual = uni(dual_) != 0;
     const bool pbc = uni(pbc_) != 0;
     const double L = uni(L_), k = uni(k_), bt = uni(bt_);
     const double* x = uni(x_);
@@ -540,14 +542,27 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     const size_t bs = (size_t)D * N;
     // beads 1 and P and the link prefix sums in LDS; cycle energies are
     // recomputed from them wherever they are needed
-    const RecShared sh = rec_carve(lds, N, D);
-    const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
-    exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh);
-    exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh);
-    __threadfence_block();
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
-    wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_shared_doubles(N, D));
+    if (dual) {
+        // the two recursions side by side: even waves forward, odd waves
+        // backward (their in-block chains then sit on different SIMDs)
+        const RecDual rd = rec_carve_dual(lds, N, D);
+        const Ring r = ring_stage<D>(rd.s0, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
+        const int wave = uni((int)(threadIdx.x / WAVE)), nw = blockDim.x / WAVE;
+        const int gtid = (wave >> 1) * WAVE + (int)(threadIdx.x % WAVE), gnthr = (nw / 2) * WAVE;
+        if ((wave & 1) == 0) exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s0, gtid, gnthr);
+        else exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s1, gtid, gnthr);
+        __threadfence_block();
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), rd.tmp);
+    } else {
+        const RecShared sh = rec_carve(lds, N, D);
+        const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
+        exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+        exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+        __threadfence_block();
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_shared_doubles(N, D));
+    }
 }
 
 // Work of one launch: a full MD step, or parts of it for timing.
@@ -575,7 +590,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.Vb + r.off_V, bp.status + ri, lds);
+                           bp.Vb + r.off_V, bp.status + ri, lds, r.xdual);
         if constexpr (integ) {
             double cm[D];
             wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm);

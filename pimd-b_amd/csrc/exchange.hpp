@@ -224,6 +224,44 @@ __device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
     return s;
 }
 
+// Workspace of the two recursions run side by side by two halves of one
+// workgroup (the batched form): the staged ring, then each direction's own
+// U, ln i, R, partials and in-block matrix.  After the recursions the
+// region from `tmp` on is free (the exterior forces' scratch).
+struct RecDual {
+    RecShared s0, s1;
+    double* tmp;
+};
+__host__ __device__ inline size_t rec_dir_doubles(int N) {
+    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
+}
+__host__ __device__ inline size_t rec_dual_doubles(int N, int D) {
+    return (size_t)(2 * D + 2) * N + 2 * rec_dir_doubles(N);
+}
+__device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D) {
+    RecDual rd;
+    double* p = base;
+    double* x1 = p; p += (size_t)D * N;
+    double* xP = p; p += (size_t)D * N;
+    double* Shi = p; p += N;
+    double* Slo = p; p += N;
+    rd.tmp = p;
+    RecShared* ss[2] = {&rd.s0, &rd.s1};
+    for (int q = 0; q < 2; ++q) {
+        RecShared& sq = *ss[q];
+        sq.U = p; p += N + 1;
+        sq.lg = p; p += N + 1;
+        sq.R = p; p += REC_B;
+        sq.part = p; p += (size_t)REC_MAXWAVES * REC_B;
+        sq.a = p; p += (size_t)REC_B * REC_B;
+        sq.x1 = x1;
+        sq.xP = xP;
+        sq.Shi = Shi;
+        sq.Slo = Slo;
+    }
+    return rd;
+}
+
 // Copies beads 1 and P into the workspace and builds the prefix sums there
 // (whole workgroup); optionally also writes the sums to global memory.
 template <int D>
@@ -284,17 +322,25 @@ __device__ __forceinline__ double rec_w(const RecShared& sh, const Ring& r, doub
     return bt * e + (DIR == 0 ? t.lg : sh.lg[N - q]);
 }
 
-// One recursion (DIR 0: forward V, DIR 1: backward Vb) for one ring, run by
-// the whole workgroup (64..1024 threads) on the staged ring (ring_stage).
-// Lane t of every wave serves target n0+t of the current block; the waves
-// split the sources of earlier blocks and keep UNROLL evaluations in flight.
-// Writes V[0..N] and Vb[0] (DIR 0) or Vb[1..N] (DIR 1); sets *status to 1
-// when a potential is not finite (the reference throws overflow_error,
+// Blocks of the recursions for N particles; the backward recursion has N-1
+// targets and may leave the last block empty.
+__host__ __device__ inline int rec_blocks(int N) { return (N + REC_B - 1) / REC_B; }
+
+// One recursion (DIR 0: forward V, DIR 1: backward Vb) for one ring on the
+// staged ring (ring_stage), run by a group of gnthr threads (a multiple of 64)
+// of the workgroup; gtid is the thread's index in its group.  Every thread of
+// the workgroup calls this (or the other direction, in a second group) and
+// passes the same workgroup barriers: one after set-up, four per block for
+// rec_blocks(N) blocks, one at the end.  Lane t of every wave serves target
+// n0+t of the current block; the group's waves split the sources of earlier
+// blocks and keep UNROLL evaluations in flight.  Writes V[0..N] and Vb[0]
+// (DIR 0) or Vb[1..N] (DIR 1); sets *status to 1 when a potential is not
+// finite (the reference throws overflow_error,
 // quadratic_bosonic_exchange.cpp:92-97, 119-124).
 template <int DIR, int D, int UNROLL = REC_UNROLL>
 __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __restrict__ V, double* __restrict__ Vb,
-                                   int* __restrict__ status, RecShared sh) {
-    const int tid = threadIdx.x, nthr = blockDim.x;
+                                   int* __restrict__ status, RecShared sh, int gtid, int gnthr) {
+    const int tid = gtid, nthr = gnthr;
     const int lane = tid % WAVE, wave = tid / WAVE;
     const int nwave = min(nthr / WAVE, REC_MAXWAVES);
     const int ntarget = DIR == 0 ? N : N - 1;
@@ -303,8 +349,17 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         sh.U[i] = 0.0;
     }
     __syncthreads();
-    for (int n0 = 1; n0 <= ntarget; n0 += REC_B) {
+    const int nblk = rec_blocks(N);
+    for (int blk = 0; blk < nblk; ++blk) {
+        const int n0 = 1 + blk * REC_B;
         const int nb = min(REC_B, ntarget - n0 + 1);
+        if (nb <= 0) {   // the backward recursion's empty last block: barriers only
+            __syncthreads();
+            __syncthreads();
+            __syncthreads();
+            __syncthreads();
+            continue;
+        }
         const int tn = n0 + min(lane, nb - 1);   // the lane's target (clamped)
         const RecTarget<D> tg = rec_target<DIR, D>(sh, N, tn);
         // (0) in-block weights w(n0+t, n0+j), j < t, staged as a[j][t]
@@ -415,8 +470,8 @@ static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, doub
     const RecShared sh = rec_carve(smem, N, D);
     const bool fwd = blockIdx.x == 0;
     const Ring r = ring_stage<D>(sh, N, x1, xP, k, pbc != 0, L, fwd ? Shi : nullptr, fwd ? Slo : nullptr);
-    if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh);
-    else exchange_recursion<1, D>(N, bt, r, V, Vb, status, sh);
+    if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+    else exchange_recursion<1, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
 }
 
 // Connection probability P[l][u] (quadratic_bosonic_exchange.cpp:142-157).

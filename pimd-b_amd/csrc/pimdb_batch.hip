@@ -217,7 +217,14 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     size_t need = 0;
     for (int r = 0; r < nrep; ++r) {
         const pimdb_params& p = reps[r];
-        need = std::max(need, rec_shared_doubles(p.natoms, p.ndim) + ext_scratch_doubles(p.natoms, p.ndim, 1024));
+        if (b->desc[r].bosonic) {
+            // both recursions side by side when their workspace fits the LDS
+            const size_t dual = rec_dual_doubles(p.natoms, p.ndim);
+            b->desc[r].xdual = sizeof(double) * (16 + dual) <= device_lds_limit() ? 1 : 0;
+            need = std::max(need, b->desc[r].xdual ? dual
+                                                   : rec_shared_doubles(p.natoms, p.ndim) +
+                                                         ext_scratch_doubles(p.natoms, p.ndim, 1024));
+        }
         need = std::max(need, batch_nm_lds_doubles(p.nbeads, 1024 / WAVE));
     }
     b->lds = sizeof(double) * (16 + need);
