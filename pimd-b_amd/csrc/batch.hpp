@@ -23,6 +23,7 @@ struct RepDesc {
     int N, P, bosonic, pbc;
     int thermostat, nm, fixcom;
     int xdual;                  // exchange recursions side by side (LDS permitting)
+    int plds;                   // pair forces accumulated in LDS (LDS permitting)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
     double int_k, strength, cutoff;
@@ -83,6 +84,7 @@ __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
     u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.xdual = uni(r.xdual);
+    u.plds = uni(r.plds);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
     u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
@@ -127,10 +129,6 @@ __host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
 // STREAM_UNROLL elements per thread per pass and issue all their loads before
 // any store, so each thread keeps that many loads in flight.
 constexpr int STREAM_UNROLL = 4;
-
-// LDS pointer type: accesses through it are ds_ instructions that the
-// compiler knows cannot alias global memory.
-typedef __attribute__((address_space(3))) double lds_double;
 
 // Axis of element e of a [bead][axis][particle] array.
 template <int D>
@@ -466,8 +464,9 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
 // (simulation.cpp:353-455).  Each wave owns whole beads: the pairs of a bead
 // with Newton's third law (slice_pair_forces), then the trap and the springs.
 template <int D, int KIND>
-__device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp) {
+__device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp, double* lds_) {
     const RepDesc r = rep_uniform(r_);
+    lds_double* F = (lds_double*)uni(lds_);
     const int N = r.N, P = r.P;
     const double* x = uni(bp.x) + r.off_x;
     double* f = uni(bp.f) + r.off_x;
@@ -476,14 +475,25 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.int_k, r.strength);
+    lds_double* Fw = F + (size_t)wave * bs;   // this wave's slice forces (r.plds)
     for (int b = wave; b < P; b += nw) {
         const double* xs = x + bs * b;
         double* fo = f + bs * b;
-        if (KIND != POT_FREE && r.cutoff != 0.0) {
-            switch (pair_image(pp)) {
-            case 0: slice_pair_forces<D, KIND, 0>(xs, fo, N, pp, lane); break;
-            case 1: slice_pair_forces<D, KIND, 1>(xs, fo, N, pp, lane); break;
-            default: slice_pair_forces<D, KIND, 2>(xs, fo, N, pp, lane); break;
+        const bool pairs = KIND != POT_FREE && r.cutoff != 0.0;
+        const bool in_lds = pairs && r.plds;
+        if (pairs) {
+            if (in_lds) {
+                switch (pair_image(pp)) {
+                case 0: slice_pair_forces_lds<D, KIND, 0>(xs, N, pp, lane, Fw); break;
+                case 1: slice_pair_forces_lds<D, KIND, 1>(xs, N, pp, lane, Fw); break;
+                default: slice_pair_forces_lds<D, KIND, 2>(xs, N, pp, lane, Fw); break;
+                }
+            } else {
+                switch (pair_image(pp)) {
+                case 0: slice_pair_forces<D, KIND, 0>(xs, fo, N, pp, lane); break;
+                case 1: slice_pair_forces<D, KIND, 1>(xs, fo, N, pp, lane); break;
+                default: slice_pair_forces<D, KIND, 2>(xs, fo, N, pp, lane); break;
+                }
             }
         } else {
             for (int e = lane; e < D * N; e += WAVE) fo[e] = 0.0;
@@ -502,7 +512,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 const int e = a * N + i;
-                const double phys = fe[a] + fo[e];
+                const double phys = fe[a] + (in_lds ? Fw[i * D + a] : fo[e]);
                 double spring = 0.0;
                 if (!exchange_bead) {
                     const double dp = disp(xv[a], xprev[e], pbc, r.L);
@@ -587,7 +597,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
             wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm);
             wg_kick_drift<D>(r, bp, cm);
         }
-        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp);
+        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
                            bp.Vb + r.off_V, bp.status + ri, lds, r.xdual);

@@ -104,6 +104,10 @@ __device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&
     return true;
 }
 
+// LDS pointer type: accesses through it are ds_ instructions that the
+// compiler knows cannot alias global memory.
+typedef __attribute__((address_space(3))) double lds_double;
+
 // Lane l receives what lane (l+1) mod 64 held (DPP wave_rol:1).  Call with
 // the whole wave active.
 __device__ __forceinline__ int lane_rotate(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
@@ -170,6 +174,63 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
         }
         __threadfence_block();
     }
+}
+
+// The same pairs with the slice's forces accumulated in LDS: F is this
+// wave's [N][D] array in LDS (particle-major, so a particle's axes sit at
+// constant offsets from one address).  The column particles' forces are added to F at
+// every step (a no-return LDS atomic add; the 64 lanes of a step hold 64
+// distinct columns, so the adds do not collide and land in step order), so
+// only the positions and the column index rotate along the wave.  The row
+// forces are added at the end of each row tile; the result stays in F (the
+// caller adds the rest of the force and writes the slice once).  Used when
+// every wave's slice fits the LDS.
+template <int D, int KIND, int IMG>
+__device__ __forceinline__ void slice_pair_forces_lds(const double* xs, int N, const PairParams& pp, int lane,
+                                                      lds_double* F) {
+    for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    const int T = (N + PTILE - 1) / PTILE;
+    for (int I = 0; I < T; ++I) {
+        const int i = I * PTILE + lane;
+        const bool row_ok = i < N;
+        double xi[D], racc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = row_ok ? xs[a * N + i] : 0.0;
+            racc[a] = 0.0;
+        }
+        for (int J = I; J < T; ++J) {
+            const bool diag = J == I;
+            const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
+            int jc = J * PTILE + ((lane + s0) & (PTILE - 1));
+            double xj[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) xj[a] = jc < N ? xs[a * N + jc] : 0.0;
+            for (int s = s0; s < s1; ++s) {
+                if (row_ok && jc < N && (!diag || s < 32 || lane < 32)) {
+                    double d[D], g;
+                    if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            racc[a] = fma(-g, d[a], racc[a]);
+                            __hip_atomic_fetch_add(&F[jc * D + a], g * d[a], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
+                jc = lane_rotate(jc);
+            }
+        }
+        if (row_ok) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                __hip_atomic_fetch_add(&F[i * D + a], racc[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 }  // namespace pimdb
