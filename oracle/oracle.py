@@ -16,6 +16,7 @@ _DP = C.POINTER(C.c_double)
 POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
 THERMO = {"none": 0, "langevin": 1}
+EXCHANGE = {"quadratic": 0, "factorial": 1}
 
 
 class Params(C.Structure):
@@ -32,6 +33,7 @@ class Params(C.Structure):
         ("seed", C.c_uint),
         ("ext_strength", C.c_double), ("ext_location", C.c_double),
         ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
+        ("exchange_algo", C.c_int),
     ]
 
 
@@ -49,6 +51,8 @@ def make_params(**kw):
             v = PROP[v]
         if k == "thermostat" and isinstance(v, str):
             v = THERMO[v]
+        if k == "exchange_algo" and isinstance(v, str):
+            v = EXCHANGE[v]
         setattr(p, k, v)
     return p
 
@@ -73,6 +77,11 @@ def _load():
         "orc_exchange": (i, [i, i, d, d, i, d, _DP, _DP, _DP, _DP,
                              _DP, _DP, _DP, _DP, _DP, _DP]),
         "orc_vb_enumerate": (d, [i, d, _DP]),
+        "orc_factorial_e_shift": (d, [i, i, d, i, d, _DP, _DP]),
+        "orc_factorial_vb": (d, [i, i, d, d, i, d, _DP, _DP]),
+        "orc_factorial_force_last": (v, [i, i, d, d, i, d, _DP, _DP, _DP, d, _DP]),
+        "orc_factorial_force_first": (v, [i, i, d, d, i, d, _DP, _DP, _DP, d, _DP]),
+        "orc_factorial_prim_estimator": (d, [i, i, i, d, d, i, d, _DP, _DP, d]),
         "orc_harmonic_grad": (v, [i, d, d, _DP, _DP]),
         "orc_harmonic_V": (d, [i, d, d, _DP]),
         "orc_dipole_grad": (v, [i, d, _DP, _DP]),
@@ -236,3 +245,20 @@ class Sim:
         if rc != 0:
             raise OracleError("observables returned %d" % rc)
         return out
+
+
+def factorial_exchange(N, D, beta_t, k, pbc, L, x_first, x_last, x_second, x_prev_last):
+    """The factorial exchange (factorial_bosonic_exchange.cpp) on AoS [N][D]
+    beads: e_shift, V_eff, forces on beads 1 and P, kinetic estimator term
+    (before the /P of the IPI convention is applied by the caller: returned
+    with P = 1)."""
+    x1, xP, x2, xPm1 = (f64(a) for a in (x_first, x_last, x_second, x_prev_last))
+    L_ = lib()
+    es = L_.orc_factorial_e_shift(N, D, k, pbc, L, ptr(x1), ptr(xP))
+    vb = L_.orc_factorial_vb(N, D, beta_t, k, pbc, L, ptr(x1), ptr(xP))
+    f1 = np.zeros((N, D))
+    fP = np.zeros((N, D))
+    L_.orc_factorial_force_first(N, D, beta_t, k, pbc, L, ptr(x1), ptr(xP), ptr(x2), es, ptr(f1))
+    L_.orc_factorial_force_last(N, D, beta_t, k, pbc, L, ptr(x1), ptr(xP), ptr(xPm1), es, ptr(fP))
+    prim = L_.orc_factorial_prim_estimator(N, 1, D, beta_t, k, pbc, L, ptr(x1), ptr(xP), es)
+    return dict(e_shift=es, vb=vb, f_first=f1, f_last=fP, prim=prim)

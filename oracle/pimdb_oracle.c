@@ -255,6 +255,141 @@ int orc_exchange(int N, int D, double beta_t, double k, int pbc, double L,
     return rc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Factorial exchange, src/bosonic_exchange/factorial_bosonic_exchange.cpp:  */
+/* the compile-time alternative over all N! permutations.  `labels` runs     */
+/* through the permutations in lexicographic order from the identity          */
+/* (std::ranges::next_permutation); the last bead of particle l connects to   */
+/* the first bead of particle labels[l] (lastBeadNeighbor, :48-50), and the   */
+/* first bead of l to the last bead of the particle at l's position in        */
+/* labels (firstBeadNeighbor, :33-35).                                        */
+
+/* std::next_permutation on a[0..n): the next permutation in lexicographic
+ * order; after the last one, a is sorted again and 0 is returned. */
+static int next_perm(int n, int* a) {
+    int i = n - 2;
+    while (i >= 0 && a[i] >= a[i + 1]) --i;
+    if (i < 0) {
+        for (int lo = 0, hi = n - 1; lo < hi; ++lo, --hi) { int t = a[lo]; a[lo] = a[hi]; a[hi] = t; }
+        return 0;
+    }
+    int j = n - 1;
+    while (a[j] <= a[i]) --j;
+    int t = a[i]; a[i] = a[j]; a[j] = t;
+    for (int lo = i + 1, hi = n - 1; lo < hi; ++lo, --hi) { t = a[lo]; a[lo] = a[hi]; a[hi] = t; }
+    return 1;
+}
+
+static int first_bead_neighbor(int n, const int* labels, int l) {
+    for (int m = 0; m < n; ++m)
+        if (labels[m] == l) return m;
+    return n;
+}
+
+/* getMinExteriorSpringEnergy, :57-79 */
+double orc_factorial_e_shift(int N, int D, double k, int pbc, double L,
+                             const double* x_first, const double* x_last) {
+    int labels[ORC_FACTORIAL_MAXN];
+    for (int i = 0; i < N; ++i) labels[i] = i;
+    double min_delta = DBL_MAX;
+    do {
+        double diff2 = 0.0;
+        for (int l = 0; l < N; ++l) diff2 += bead_sep_sq(N, D, x_first, l, x_last, labels[l], pbc, L);
+        min_delta = fmin(min_delta, 0.5 * k * diff2);
+    } while (next_perm(N, labels));
+    return min_delta;
+}
+
+/* effectivePotential, :88-114 (beta_half_k = thermo_beta * 0.5 * k,
+ * simulation.cpp:52) */
+double orc_factorial_vb(int N, int D, double beta_t, double k, int pbc, double L,
+                        const double* x_first, const double* x_last) {
+    int labels[ORC_FACTORIAL_MAXN];
+    for (int i = 0; i < N; ++i) labels[i] = i;
+    const double beta_half_k = beta_t * 0.5 * k;
+    long count = 0;
+    double weights_sum = 0.0;
+    do {
+        count++;
+        double diff2 = 0.0;
+        for (int l = 0; l < N; ++l) diff2 += bead_sep_sq(N, D, x_first, l, x_last, labels[l], pbc, L);
+        weights_sum += exp(-beta_half_k * diff2);
+    } while (next_perm(N, labels));
+    return (-1.0 / beta_t) * log(weights_sum / count);
+}
+
+/* springForceLastBead, :121-172: x_last = this bead (P), x_prev = bead P-1,
+ * x_first = bead 1 (x_next of bead P). */
+void orc_factorial_force_last(int N, int D, double beta_t, double k, int pbc, double L,
+                              const double* x_first, const double* x_last, const double* x_prev,
+                              double e_shift, double* f) {
+    int labels[ORC_FACTORIAL_MAXN];
+    double temp[ORC_FACTORIAL_MAXN * 8];
+    for (int i = 0; i < N; ++i) labels[i] = i;
+    for (int e = 0; e < N * D; ++e) f[e] = 0.0;
+    double denom = 0.0;
+    do {
+        double weight = 0.0;
+        for (int l = 0; l < N; ++l) {
+            double dn[8], dp[8];
+            bead_sep(N, D, x_last, l, x_first, labels[l], pbc, L, dn);
+            for (int a = 0; a < D; ++a) weight += dn[a] * dn[a];
+            bead_sep(N, D, x_last, l, x_prev, l, pbc, L, dp);
+            for (int a = 0; a < D; ++a) temp[l * D + a] = (0.0 + dp[a] + dn[a]) * k;
+        }
+        weight = exp(-beta_t * (0.5 * k * weight - e_shift));
+        for (int e = 0; e < N * D; ++e) f[e] += weight * temp[e];
+        denom += weight;
+    } while (next_perm(N, labels));
+    for (int e = 0; e < N * D; ++e) f[e] = f[e] / denom;
+}
+
+/* springForceFirstBead, :179-230: x_first = this bead (1), x_last = bead P
+ * (x_prev of bead 1), x_next = bead 2. */
+void orc_factorial_force_first(int N, int D, double beta_t, double k, int pbc, double L,
+                               const double* x_first, const double* x_last, const double* x_next,
+                               double e_shift, double* f) {
+    int labels[ORC_FACTORIAL_MAXN];
+    double temp[ORC_FACTORIAL_MAXN * 8];
+    for (int i = 0; i < N; ++i) labels[i] = i;
+    for (int e = 0; e < N * D; ++e) f[e] = 0.0;
+    double denom = 0.0;
+    do {
+        double weight = 0.0;
+        for (int l = 0; l < N; ++l) {
+            double dp[8], dn[8];
+            bead_sep(N, D, x_first, l, x_last, first_bead_neighbor(N, labels, l), pbc, L, dp);
+            for (int a = 0; a < D; ++a) weight += dp[a] * dp[a];
+            bead_sep(N, D, x_first, l, x_next, l, pbc, L, dn);
+            for (int a = 0; a < D; ++a) temp[l * D + a] = (0.0 + dp[a] + dn[a]) * k;
+        }
+        weight = exp(-beta_t * (0.5 * k * weight - e_shift));
+        for (int e = 0; e < N * D; ++e) f[e] += weight * temp[e];
+        denom += weight;
+    } while (next_perm(N, labels));
+    for (int e = 0; e < N * D; ++e) f[e] = f[e] / denom;
+}
+
+/* primEstimator, :258-290 (IPI convention: divided by P) */
+double orc_factorial_prim_estimator(int N, int P, int D, double beta_t, double k, int pbc, double L,
+                                    const double* x_first, const double* x_last, double e_shift) {
+    int labels[ORC_FACTORIAL_MAXN];
+    for (int i = 0; i < N; ++i) labels[i] = i;
+    double numerator = 0.0, denom = 0.0;
+    do {
+        double weight = 0.0;
+        for (int l = 0; l < N; ++l)
+            weight += bead_sep_sq(N, D, x_first, l, x_last, first_bead_neighbor(N, labels, l), pbc, L);
+        const double delta = 0.5 * k * weight;
+        weight = exp(-beta_t * (delta - e_shift));
+        numerator += delta * weight;
+        denom += weight;
+    } while (next_perm(N, labels));
+    double e = numerator / denom;
+    e /= P;
+    return (-1.0) * e;
+}
+
 /* Brute force over compositions (independent of the recursion above): every
  * bit pattern of N-1 "cuts" splits 0..N-1 into consecutive blocks [s..e]; a
  * block weighs exp(-beta E^(s,e)) / (e+1) with E^(s,e) = E(e+1, e-s+1). */
@@ -537,9 +672,24 @@ int orc_update_forces(const orc_params* p, const double* x, double* f) {
     if (is_bosonic(p)) {
         f_first = (double*)malloc(sizeof(double) * nb);
         f_last = (double*)malloc(sizeof(double) * nb);
-        rc = orc_exchange(N, D, d.beta_t, d.k, p->pbc, p->size,
-                          x, x + nb * 1, x + nb * (P - 2), x + nb * (P - 1),
-                          NULL, NULL, NULL, NULL, f_first, f_last);
+        if (p->exchange_algo == ORC_EXCHANGE_FACTORIAL) {
+            if (N > ORC_FACTORIAL_MAXN) {
+                rc = ORC_E_ARG;
+            } else {
+                /* prepare() then exteriorSpringForce, simulation.cpp:396-401 */
+                const double* x1 = x;
+                const double* xP = x + nb * (P - 1);
+                const double e_shift = orc_factorial_e_shift(N, D, d.k, p->pbc, p->size, x1, xP);
+                orc_factorial_force_first(N, D, d.beta_t, d.k, p->pbc, p->size, x1, xP, x + nb * 1, e_shift,
+                                          f_first);
+                orc_factorial_force_last(N, D, d.beta_t, d.k, p->pbc, p->size, x1, xP, x + nb * (P - 2), e_shift,
+                                         f_last);
+            }
+        } else {
+            rc = orc_exchange(N, D, d.beta_t, d.k, p->pbc, p->size,
+                              x, x + nb * 1, x + nb * (P - 2), x + nb * (P - 1),
+                              NULL, NULL, NULL, NULL, f_first, f_last);
+        }
     }
     if (rc == ORC_OK) {
         for (int b = 0; b < P; ++b) {
@@ -960,7 +1110,9 @@ int orc_sim_observables(orc_sim* s, double* out) {
     const double cutoff = orc_effective_cutoff(p);
     double *E = NULL, *V = NULL;
     int rc = ORC_OK;
-    if (is_bosonic(p)) {
+    const int factorial = is_bosonic(p) && p->exchange_algo == ORC_EXCHANGE_FACTORIAL;
+    if (factorial && N > ORC_FACTORIAL_MAXN) return ORC_E_ARG;
+    if (is_bosonic(p) && !factorial) {
         E = (double*)malloc(sizeof(double) * (size_t)N * (N + 1) / 2);
         V = (double*)malloc(sizeof(double) * (N + 1));
         double* Vb = (double*)malloc(sizeof(double) * (N + 1));
@@ -984,7 +1136,11 @@ int orc_sim_observables(orc_sim* s, double* out) {
             }
         spring_b *= 0.5 * s->d.k;
         double kin = 0.5 * D * N / s->d.beta;
-        if (b == 0 && is_bosonic(p)) kin += orc_prim_estimator(N, P, s->d.beta_t, E, V);
+        if (b == 0 && factorial) {
+            const double* xP = s->x + nb * (P - 1);
+            const double e_shift = orc_factorial_e_shift(N, D, s->d.k, p->pbc, p->size, s->x, xP);
+            kin += orc_factorial_prim_estimator(N, P, D, s->d.beta_t, s->d.k, p->pbc, p->size, s->x, xP, e_shift);
+        } else if (b == 0 && is_bosonic(p)) kin += orc_prim_estimator(N, P, s->d.beta_t, E, V);
         else kin -= spring_b / P;
         kinetic += kin;
         double pot = 0, virial = 0, int_pot = 0, ext_pot = 0;
@@ -1034,10 +1190,13 @@ int orc_sim_observables(orc_sim* s, double* out) {
         double temp_b = 2.0 * ke / (dof * 1.0);
         temp_b /= P;
         temperature += temp_b;
-        if (b == 0 && is_bosonic(p)) cl_spring += V[N];
+        if (b == 0 && factorial)
+            cl_spring += orc_factorial_vb(N, D, s->d.beta_t, s->d.k, p->pbc, p->size, s->x, s->x + nb * (P - 1));
+        else if (b == 0 && is_bosonic(p)) cl_spring += V[N];
         else cl_spring += spring_b;
     }
-    if (is_bosonic(p)) {
+    /* the factorial exchange returns 0 for both (factorial_bosonic_exchange.cpp:237-253) */
+    if (is_bosonic(p) && !factorial) {
         prob_dist = orc_distinct_probability(N, s->d.beta_t, E, V);
         prob_all = orc_longest_probability(N, s->d.beta_t, E, V);
     }

@@ -51,7 +51,25 @@ typedef struct orc_params {
     /* external double well (strength: energy, location: length) and cosine
      * (amplitude: energy, phase: number; the wavelength is the box size) */
     double ext_strength, ext_location, ext_amplitude, ext_phase;
+    int exchange_algo;  /* ORC_EXCHANGE_QUADRATIC or _FACTORIAL (FACTORIAL_BOSONIC_ALGORITHM) */
 } orc_params;
+
+enum { ORC_EXCHANGE_QUADRATIC = 0, ORC_EXCHANGE_FACTORIAL = 1 };
+#define ORC_FACTORIAL_MAXN 10
+
+/* ---- factorial exchange (src/bosonic_exchange/factorial_bosonic_exchange.cpp) ---- */
+double orc_factorial_e_shift(int N, int D, double k, int pbc, double L,
+                             const double* x_first, const double* x_last);
+double orc_factorial_vb(int N, int D, double beta_t, double k, int pbc, double L,
+                        const double* x_first, const double* x_last);
+void orc_factorial_force_last(int N, int D, double beta_t, double k, int pbc, double L,
+                              const double* x_first, const double* x_last, const double* x_prev,
+                              double e_shift, double* f);
+void orc_factorial_force_first(int N, int D, double beta_t, double k, int pbc, double L,
+                               const double* x_first, const double* x_last, const double* x_next,
+                               double e_shift, double* f);
+double orc_factorial_prim_estimator(int N, int P, int D, double beta_t, double k, int pbc, double L,
+                                    const double* x_first, const double* x_last, double e_shift);
 
 /* ---- exchange (src/bosonic_exchange/quadratic_bosonic_exchange.cpp) ---- */
 double orc_min_image(double dx, double L);

@@ -25,8 +25,12 @@ DYNAMICS = [
     "bosonic_quadratic_harmonic_nmthermostat_dynamics",
     "dist_harmonic_dynamics",
     "dist_harmonic_nm_propagation_dynamics",
+    "bosonic_factorial_harmonic_dynamics",
+    "bosonic_quadratic_harmonic_nh_dynamics",
+    "bosonic_quadratic_harmonic_nh_np_dynamics",
+    "bosonic_quadratic_harmonic_nh_np_dim_dynamics",
 ]
-OUT_ONLY = ["bosonic_quadratic_harmonic", "dist_harmonic"]
+OUT_ONLY = ["bosonic_quadratic_harmonic", "dist_harmonic", "bosonic_factorial_harmonic"]
 
 
 def read_frames(path, nframes):

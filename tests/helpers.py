@@ -52,6 +52,10 @@ def case_params(case):
         propagator=sim.get("propagator", "cartesian"), thermostat=sim.get("thermostat"),
         nmthermostat=int(b(sim.get("nmthermostat"), False)), fixcom=int(b(sim.get("fixcom"), True)),
         seed=int(float(sim.get("seed", "1234"))),
+        # the reference picks the exchange algorithm at build time
+        # (FACTORIAL_BOSONIC_ALGORITHM); its harness runs the *factorial* cases
+        # with that build (tests/main.py:286-290)
+        exchange_algo=1 if "factorial" in case else 0,
     ), int(sim.get("sfreq", "1000"))
 
 

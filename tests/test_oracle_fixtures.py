@@ -12,8 +12,8 @@ from helpers import (ANGSTROM, FORCE_EV_A, KELVIN, VEL_A_PS, case_params, load_f
                      per_bead_err, rel_err)
 
 DYN = ["bosonic_quadratic_harmonic_dynamics", "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-       "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics"]
-OUT = DYN + ["bosonic_quadratic_harmonic", "dist_harmonic"]
+       "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics"]
+OUT = DYN + ["bosonic_quadratic_harmonic", "dist_harmonic", "bosonic_factorial_harmonic"]
 
 
 @pytest.mark.parametrize("case", DYN)

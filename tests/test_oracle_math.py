@@ -339,3 +339,80 @@ def test_physical_forces_use_external_gradient():
             else:
                 O.lib().orc_cosine_grad(3, 0.02, 7.5, 0.3, O.ptr(x[i].copy()), O.ptr(g))
             assert np.array_equal(f[i], -g)
+
+
+# ---- factorial exchange (factorial_bosonic_exchange.cpp) ----
+
+def _perm_energies(x1, xP, k, pbc, L):
+    """0.5 k sum_l |x_P[sigma(l)] - x1[l]|^2 for every permutation sigma,
+    by itertools (independent of the oracle's next_permutation walk)."""
+    import itertools
+    N = x1.shape[0]
+    out = []
+    for s in itertools.permutations(range(N)):
+        d = xP[list(s)] - x1
+        if pbc:
+            d = d - L * np.floor(d / L + 0.5)
+        out.append(0.5 * k * np.sum(d * d))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_factorial_vb_and_shift_brute_force(pbc):
+    """V_eff = -(1/beta~) ln(mean_sigma exp(-beta~ E_sigma)) and e_shift =
+    min_sigma E_sigma against an itertools enumeration."""
+    N, D, bt, k, L = 5, 3, 0.8, 0.7, 4.0
+    x1, x2, xPm1, xP = ring(N, D, 31, scale=1.1)
+    out = O.factorial_exchange(N, D, bt, k, int(pbc), L, x1, xP, x2, xPm1)
+    E = _perm_energies(x1, xP, k, pbc, L)
+    assert abs(out["e_shift"] - E.min()) <= 1e-14 * E.min()
+    vb = E.min() - np.log(np.mean(np.exp(-bt * (E - E.min())))) / bt
+    assert abs(out["vb"] - vb) <= 1e-12 * abs(vb)
+    # kinetic term: -<E_sigma> over the Boltzmann weights (P = 1 here)
+    w = np.exp(-bt * (E - E.min()))
+    assert abs(out["prim"] + np.sum(E * w) / np.sum(w)) <= 1e-12 * np.sum(E * w) / np.sum(w)
+
+
+def test_factorial_equals_quadratic_for_two_particles():
+    """For N = 2 the permutation average and the Feldman-Hirshberg recursion
+    are the same potential (V_B, forces and the kinetic term agree)."""
+    N, D, bt, k = 2, 3, 1.3, 0.9
+    x1, x2, xPm1, xP = ring(N, D, 41, scale=0.9)
+    fa = O.factorial_exchange(N, D, bt, k, 0, 0.0, x1, xP, x2, xPm1)
+    fh = O.exchange(x1, x2, xPm1, xP, bt, k)
+    assert abs(fa["vb"] - fh["V"][N]) <= 1e-12 * abs(fh["V"][N])
+    assert np.max(np.abs(fa["f_first"] - fh["f_first"])) <= 1e-12 * np.max(np.abs(fh["f_first"]))
+    assert np.max(np.abs(fa["f_last"] - fh["f_last"])) <= 1e-12 * np.max(np.abs(fh["f_last"]))
+    E = O.cycle_energies(x1, xP, k)
+    prim_fh = O.lib().orc_prim_estimator(N, 1, bt, O.ptr(E), O.ptr(fh["V"]))
+    assert abs(fa["prim"] - prim_fh) <= 1e-12 * abs(prim_fh)
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_factorial_forces_are_minus_gradient(pbc):
+    """F1, FP = -grad(V_eff + interior links) by central differences."""
+    N, D, bt, k, L = 4, 3, 1.1, 0.9, 5.0
+    x1, x2, xPm1, xP = ring(N, D, 51, scale=1.2)
+
+    def energy(a1, aP):
+        V = O.factorial_exchange(N, D, bt, k, int(pbc), L, a1, aP, x2, xPm1)["vb"]
+        d1 = x2 - a1
+        dP = xPm1 - aP
+        if pbc:
+            d1 = d1 - L * np.floor(d1 / L + 0.5)
+            dP = dP - L * np.floor(dP / L + 0.5)
+        return V + 0.5 * k * np.sum(d1 ** 2) + 0.5 * k * np.sum(dP ** 2)
+
+    out = O.factorial_exchange(N, D, bt, k, int(pbc), L, x1, xP, x2, xPm1)
+    h = 1e-6
+    for which, f in ((0, out["f_first"]), (1, out["f_last"])):
+        for i in range(N):
+            for a in range(D):
+                xa, xb = x1.copy(), xP.copy()
+                tgt = xa if which == 0 else xb
+                tgt[i, a] += h
+                ep = energy(xa, xb)
+                tgt[i, a] -= 2 * h
+                em = energy(xa, xb)
+                fd = -(ep - em) / (2 * h)
+                assert abs(fd - f[i, a]) < 2e-7 * max(1.0, abs(f[i, a])), (which, i, a, fd, f[i, a])
