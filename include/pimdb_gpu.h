@@ -63,7 +63,15 @@ typedef struct pimdb_params {
      * cosine: amplitude (energy), phase (number), wavelength = size
      * (double_well.cpp:3-4, cosine.cpp:4-7, simulation.cpp:624-637) */
     double ext_strength, ext_location, ext_amplitude, ext_phase;
+    /* bosonic exchange algorithm: PIMDB_EXCHANGE_QUADRATIC (Feldman-Hirshberg,
+     * quadratic_bosonic_exchange.cpp) or PIMDB_EXCHANGE_FACTORIAL (all N!
+     * permutations, factorial_bosonic_exchange.cpp, natoms <= 10); the
+     * reference chooses between them at build time
+     * (FACTORIAL_BOSONIC_ALGORITHM, simulation.cpp:694-698) */
+    int exchange_algo;
 } pimdb_params;
+
+enum pimdb_exchange_algo { PIMDB_EXCHANGE_QUADRATIC = 0, PIMDB_EXCHANGE_FACTORIAL = 1 };
 
 typedef struct pimdb_ctx pimdb_ctx;
 

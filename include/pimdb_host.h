@@ -48,9 +48,12 @@ int pimdb_host_draw_noise(const char* path, long nsteps, double* out, char* err,
 
 /* Simulation::run (src/simulation.cpp:222-290) on the GPU: writes
  * simulation.out, report.txt and the enabled position/velocity/force files
- * under output_dir.  steps >= 0 replaces the configured step count;
- * device_noise selects on-device Philox noise instead of the RanMars streams. */
-int pimdb_host_run(const char* path, long steps, int device_noise, const char* output_dir, char* err, int errlen);
+ * under output_dir.  steps >= 0 replaces the configured step count.  flags:
+ * PIMDB_HOST_DEVICE_NOISE selects on-device Philox noise instead of the
+ * RanMars streams; PIMDB_HOST_FACTORIAL the factorial bosonic exchange (the
+ * reference's FACTORIAL_BOSONIC_ALGORITHM build, simulation.cpp:694-698). */
+enum { PIMDB_HOST_DEVICE_NOISE = 1, PIMDB_HOST_FACTORIAL = 2 };
+int pimdb_host_run(const char* path, long steps, int flags, const char* output_dir, char* err, int errlen);
 
 #ifdef __cplusplus
 }

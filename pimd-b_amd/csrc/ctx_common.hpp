@@ -83,6 +83,10 @@ inline bool validate_params(const pimdb_params& p, std::string* why) {
     if (p.thermostat != PIMDB_THERMO_NONE && p.thermostat != PIMDB_THERMO_LANGEVIN) return bad("unknown thermostat");
     if (p.nmthermostat && p.thermostat == PIMDB_THERMO_NONE) return bad("nmthermostat cannot be used in nve ensemble!");
     if (p.nbeads > 2048) return bad("nbeads above 2048 is not supported");
+    if (p.exchange_algo != PIMDB_EXCHANGE_QUADRATIC && p.exchange_algo != PIMDB_EXCHANGE_FACTORIAL)
+        return bad("unknown bosonic exchange algorithm");
+    if (p.exchange_algo == PIMDB_EXCHANGE_FACTORIAL && p.bosonic && p.natoms > 10)
+        return bad("the factorial exchange enumerates natoms! permutations: natoms <= 10");
     return true;
 }
 

@@ -170,6 +170,10 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             set_create_error("replica " + std::to_string(r) + ": " + why);
             return PIMDB_E_ARG;
         }
+        if (reps[r].bosonic && reps[r].exchange_algo != PIMDB_EXCHANGE_QUADRATIC) {
+            set_create_error("batched replicas use the Feldman-Hirshberg exchange");
+            return PIMDB_E_ARG;
+        }
         if (reps[r].propagator != PIMDB_PROP_CARTESIAN) {
             set_create_error("batched replicas use the cartesian propagator");
             return PIMDB_E_ARG;

@@ -21,6 +21,7 @@
 #include "pimdb_gpu.h"
 #include "ctx_common.hpp"
 #include "exchange.hpp"
+#include "factorial.hpp"
 #include "forces.hpp"
 #include "integrator.hpp"
 #include "observables.hpp"
@@ -123,9 +124,37 @@ static void launch_exchange_forces(pimdb_ctx* ctx, hipStream_t s, double* f, dou
     }
 }
 
+static bool factorial(const pimdb_ctx* ctx) {
+    return ctx->dv.bosonic && ctx->prm.exchange_algo == PIMDB_EXCHANGE_FACTORIAL;
+}
+
+// The factorial exchange (one wave): forces added into f1 / fP, or written to
+// fex; V_eff to VN; the estimator triple to est.
+static void launch_factorial(pimdb_ctx* ctx, hipStream_t s, double* f, double* fex, double* VN, double* est) {
+    const int N = ctx->N, P = ctx->P, D = ctx->D;
+    const size_t bs = (size_t)D * N;
+    const double* x1 = ctx->x;
+    const double* x2 = ctx->x + bs * (1 % P);
+    const double* xPm1 = ctx->x + bs * ((P - 2 + P) % P);
+    const double* xP = ctx->x + bs * (P - 1);
+    double* f1 = f ? f : nullptr;
+    double* fP = f ? f + bs * (P - 1) : nullptr;
+    const size_t lds = sizeof(double) * factorial_lds_doubles(N, D);
+    const double bt = ctx->dv.bt, k = ctx->dv.k, L = ctx->prm.size;
+    const int pbc = ctx->prm.pbc;
+    switch (D) {
+    case 1: hipLaunchKernelGGL(k_factorial_exchange<1>, dim3(1), dim3(FACT_LANES), lds, s, N, P, bt, k, pbc, L, x1, x2, xPm1, xP, f1, fP, fex, VN, est); break;
+    case 2: hipLaunchKernelGGL(k_factorial_exchange<2>, dim3(1), dim3(FACT_LANES), lds, s, N, P, bt, k, pbc, L, x1, x2, xPm1, xP, f1, fP, fex, VN, est); break;
+    default: hipLaunchKernelGGL(k_factorial_exchange<3>, dim3(1), dim3(FACT_LANES), lds, s, N, P, bt, k, pbc, L, x1, x2, xPm1, xP, f1, fP, fex, VN, est); break;
+    }
+}
+
 // Forces of every bead (updateForces on all ranks); f and (NM propagator) fext.
 static int enqueue_forces(pimdb_ctx* ctx) {
-    if (ctx->dv.bosonic) {
+    if (factorial(ctx)) {
+        launch_bead_forces(ctx, ctx->s0);
+        launch_factorial(ctx, ctx->s0, ctx->f, nullptr, nullptr, nullptr);
+    } else if (ctx->dv.bosonic) {
         CK(hipEventRecord(ctx->ev_fork, ctx->s0));
         CK(hipStreamWaitEvent(ctx->s1, ctx->ev_fork, 0));
         launch_exchange_forward(ctx, ctx->s1);
@@ -305,7 +334,10 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
             hipMemcpy(ctx->rot, rot.data(), sizeof(double) * rot.size(), hipMemcpyHostToDevice) != hipSuccess)
             rc = ctx->fail(PIMDB_E_HIP, "upload of normal-mode tables failed");
     }
-    if (!rc && ctx->dv.bosonic && rec_lds_bytes(N, D) > device_lds_limit())
+    if (!rc && ctx->dv.bosonic && prm->exchange_algo == PIMDB_EXCHANGE_FACTORIAL &&
+        sizeof(double) * factorial_lds_doubles(N, D) > device_lds_limit())
+        rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the factorial exchange workgroup");
+    if (!rc && ctx->dv.bosonic && prm->exchange_algo != PIMDB_EXCHANGE_FACTORIAL && rec_lds_bytes(N, D) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the exchange workgroup");
     if (!rc && sizeof(double) * ((size_t)D * N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the force workgroup");
@@ -383,6 +415,16 @@ int pimdb_exchange_forward(pimdb_ctx* ctx, double* VB, double* V, double* Vb, do
     if (!ctx) return PIMDB_E_ARG;
     if (!ctx->dv.bosonic) return ctx->fail(PIMDB_E_STATE, "exchange requires bosonic=true and nbeads > 1");
     const int N = ctx->N;
+    if (factorial(ctx)) {
+        if (V || Vb || prob || E)
+            return ctx->fail(PIMDB_E_ARG, "the factorial exchange has no V, Vb, E or connection probabilities");
+        launch_factorial(ctx, ctx->s0, nullptr, nullptr, ctx->V + N, nullptr);
+        CK(hipGetLastError());
+        const int rc = check_status(ctx);   // waits for the stream
+        if (rc) return rc;
+        if (VB) CK(hipMemcpy(VB, ctx->V + N, sizeof(double), hipMemcpyDeviceToHost));
+        return PIMDB_OK;
+    }
     launch_exchange_forward(ctx, ctx->s0);
     if (prob || E) launch_cycle_energies_ctx(ctx, ctx->s0);
     if (prob) {
@@ -404,8 +446,12 @@ int pimdb_exchange_forward(pimdb_ctx* ctx, double* VB, double* V, double* Vb, do
 int pimdb_exchange_adjoint(pimdb_ctx* ctx, double* f_first, double* f_last) {
     if (!ctx) return PIMDB_E_ARG;
     if (!ctx->dv.bosonic) return ctx->fail(PIMDB_E_STATE, "exchange requires bosonic=true and nbeads > 1");
-    launch_exchange_forward(ctx, ctx->s0);
-    launch_exchange_forces(ctx, ctx->s0, nullptr, ctx->fex);
+    if (factorial(ctx)) {
+        launch_factorial(ctx, ctx->s0, nullptr, ctx->fex, nullptr, nullptr);
+    } else {
+        launch_exchange_forward(ctx, ctx->s0);
+        launch_exchange_forces(ctx, ctx->s0, nullptr, ctx->fex);
+    }
     CK(hipGetLastError());
     int rc = check_status(ctx);
     if (rc) return rc;
@@ -490,7 +536,9 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
     double* prim = epart + 3 * (size_t)P * nt;   // [3]
     launch_bead_sums(D, N, P, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, ctx->x, ctx->p, sums, ctx->s0);
     launch_bead_energies(D, ctx->prm, ctx->dv, ctx->x, epart, ctx->s0);
-    if (ctx->dv.bosonic) {
+    if (factorial(ctx)) {
+        launch_factorial(ctx, ctx->s0, nullptr, nullptr, ctx->V + N, prim);
+    } else if (ctx->dv.bosonic) {
         launch_exchange_forward(ctx, ctx->s0);
         launch_cycle_energies_ctx(ctx, ctx->s0);
         hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
@@ -514,10 +562,14 @@ int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinc
     const int N = ctx->N, P = ctx->P;
     const int nt = (N + FORCE_TILE - 1) / FORCE_TILE;
     double* out = ctx->obs + 2 * (size_t)P + 3 * (size_t)P * nt;   // [3], as pimdb_observables
-    launch_exchange_forward(ctx, ctx->s0);
-    launch_cycle_energies_ctx(ctx, ctx->s0);
-    hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
-                       ctx->dv.bt, ctx->E, ctx->V, out);
+    if (factorial(ctx)) {
+        launch_factorial(ctx, ctx->s0, nullptr, nullptr, nullptr, out);
+    } else {
+        launch_exchange_forward(ctx, ctx->s0);
+        launch_cycle_energies_ctx(ctx, ctx->s0);
+        hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
+                           ctx->dv.bt, ctx->E, ctx->V, out);
+    }
     CK(hipGetLastError());
     int rc = check_status(ctx);
     if (rc) return rc;
@@ -580,8 +632,12 @@ int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_e
         *ms_exchange = 0.0;
         if (ctx->dv.bosonic) {
             rc = time_captured(ctx, reps, [&]() {
-                launch_exchange_forward(ctx, ctx->s0);
-                launch_exchange_forces(ctx, ctx->s0, nullptr, ctx->fex);
+                if (factorial(ctx)) {
+                    launch_factorial(ctx, ctx->s0, nullptr, ctx->fex, nullptr, nullptr);
+                } else {
+                    launch_exchange_forward(ctx, ctx->s0);
+                    launch_exchange_forces(ctx, ctx->s0, nullptr, ctx->fex);
+                }
                 return PIMDB_OK;
             }, ms_exchange);
             if (rc) return rc;

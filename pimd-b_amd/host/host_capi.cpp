@@ -103,13 +103,14 @@ int pimdb_host_draw_noise(const char* path, long nsteps, double* out, char* err,
     });
 }
 
-int pimdb_host_run(const char* path, long steps, int device_noise, const char* output_dir, char* err, int errlen) {
+int pimdb_host_run(const char* path, long steps, int flags, const char* output_dir, char* err, int errlen) {
     if (!path) return PIMDB_E_ARG;
     return guarded(err, errlen, [&] {
         Params params(path);
         RunOptions o;
         o.steps_override = steps;
-        o.device_noise = device_noise != 0;
+        o.device_noise = (flags & PIMDB_HOST_DEVICE_NOISE) != 0;
+        o.factorial = (flags & PIMDB_HOST_FACTORIAL) != 0;
         if (output_dir) o.output_dir = output_dir;
         Simulation sim(params, o);
         sim.run();

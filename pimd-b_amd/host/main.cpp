@@ -1,7 +1,10 @@
 // main.cpp -- pimdb_gpu, the command-line driver (the reference's pimdb,
 // src/pimdb.cpp:6-67) running the simulation on one MI355X.
 //
-//   pimdb_gpu [-in config.ini] [--dim] [--bosonic_alg] [--steps N] [--device-noise]
+//   pimdb_gpu [-in config.ini] [--dim] [--bosonic_alg] [--steps N] [--device-noise] [--factorial]
+//
+// --factorial runs the factorial bosonic exchange, which the reference
+// selects at build time (FACTORIAL_BOSONIC_ALGORITHM).
 //
 // Errors are printed with the reference's labels; unlike the reference
 // (which always exits 0) a failed run exits with status 1.
@@ -26,7 +29,7 @@ int main(int argc, char** argv) {
                 std::printf("Program runs 1-, 2- and 3-dimensional systems (the [simulation] key ndim, default 3)\n");
                 info = true;
             } else if (!std::strcmp(argv[i], "--bosonic_alg")) {
-                std::printf("Program was compiled with quadratic bosonic algorithm.\n");
+                std::printf("Program runs the quadratic bosonic algorithm (the factorial one with --factorial).\n");
                 info = true;
             } else if (!std::strcmp(argv[i], "-in")) {
                 if (i + 1 >= argc) throw std::invalid_argument("-in option requires a filename argument");
@@ -36,6 +39,8 @@ int main(int argc, char** argv) {
                 opts.steps_override = std::atol(argv[++i]);
             } else if (!std::strcmp(argv[i], "--device-noise")) {
                 opts.device_noise = true;
+            } else if (!std::strcmp(argv[i], "--factorial")) {
+                opts.factorial = true;
             }
         }
         if (info) return 0;

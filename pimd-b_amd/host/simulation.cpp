@@ -201,6 +201,7 @@ Simulation::Simulation(Params& params, const RunOptions& options) : opts_(option
     gp_.cutoff = interaction_potential_name == "free" ? 0.0 : params.get<double>(params.interaction_pot, "cutoff");
     gp_.fixcom = fixcom;
     gp_.seed = params_seed;
+    gp_.exchange_algo = opts_.factorial ? PIMDB_EXCHANGE_FACTORIAL : PIMDB_EXCHANGE_QUADRATIC;
     ext_potential_->configure(gp_, true);
     int_potential_->configure(gp_, false);
     propagator_->configure(gp_);
@@ -500,7 +501,7 @@ void Simulation::write_report(double wall_time) const {
     rep << "---------\nParameters\n---------\n";
     if (bosonic) {
         rep << report_line("Statistics", "Bosonic");
-        rep << report_line("Bosonic algorithm", "Feldman-Hirshberg");
+        rep << report_line("Bosonic algorithm", opts_.factorial ? "Naive" : "Feldman-Hirshberg");
     } else {
         rep << report_line("Statistics", "Boltzmannonic");
     }

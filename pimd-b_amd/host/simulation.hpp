@@ -130,6 +130,7 @@ private:
 // Options of a run that the configuration file does not carry.
 struct RunOptions {
     bool device_noise = false;   // Philox noise on the GPU instead of the RanMars streams
+    bool factorial = false;      // the factorial bosonic exchange (FACTORIAL_BOSONIC_ALGORITHM)
     long steps_override = -1;    // replaces [simulation] steps when >= 0
     std::string output_dir = "output";
     // initial conditions and noise streams only: no GPU context, no files

@@ -17,6 +17,7 @@ OK, E_NONFINITE, E_ARG, E_HIP, E_STATE = 0, 1, 2, 3, 4
 POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
 THERMO = {"none": 0, "langevin": 1}
+EXCHANGE = {"quadratic": 0, "factorial": 1}
 
 
 class PimdbError(RuntimeError):
@@ -43,6 +44,7 @@ class Params(C.Structure):
         ("seed", C.c_uint),
         ("ext_strength", C.c_double), ("ext_location", C.c_double),
         ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
+        ("exchange_algo", C.c_int),
     ]
 
 
@@ -58,6 +60,8 @@ def make_params(**kw):
             v = PROP[v]
         elif k == "thermostat" and isinstance(v, str):
             v = THERMO[v]
+        elif k == "exchange_algo" and isinstance(v, str):
+            v = EXCHANGE[v]
         setattr(p, k, v)
     return p
 
@@ -169,11 +173,14 @@ class Context:
     def forces(self):
         self._chk(lib().pimdb_forces(self.h))
 
-    def exchange_forward(self, prob=True, E=True):
+    def exchange_forward(self, prob=True, E=True, V=True):
+        """V_B, and unless switched off V, Vb (V=False drops both), the
+        connection probabilities and the packed E_kn triangle.  The factorial
+        exchange defines only V_B."""
         N = self.params.natoms
         VB = np.zeros(1)
-        V = np.zeros(N + 1)
-        Vb = np.zeros(N + 1)
+        Vb = np.zeros(N + 1) if V else None
+        V = np.zeros(N + 1) if V else None
         P = np.zeros((N, N)) if prob else None
         Ek = np.zeros(N * (N + 1) // 2) if E else None
         self._chk(lib().pimdb_exchange_forward(self.h, _p(VB), _p(V), _p(Vb), _p(P), _p(Ek)))

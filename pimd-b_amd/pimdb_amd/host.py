@@ -104,5 +104,8 @@ def draw_noise(path, nsteps):
     return out
 
 
-def run(path, steps=-1, device_noise=False, output_dir="output"):
-    _call(hlib().pimdb_host_run, os.fsencode(path), int(steps), int(bool(device_noise)), os.fsencode(output_dir))
+def run(path, steps=-1, device_noise=False, output_dir="output", factorial=False):
+    """Simulation::run on the GPU (see pimdb_host_run); factorial selects the
+    factorial bosonic exchange."""
+    flags = (1 if device_noise else 0) | (2 if factorial else 0)
+    _call(hlib().pimdb_host_run, os.fsencode(path), int(steps), flags, os.fsencode(output_dir))

@@ -36,7 +36,7 @@ def run_pair(kw, nsteps, x=None, p=None, init_random=False, forces_first=False):
 
 
 CASES = ["bosonic_quadratic_harmonic_dynamics", "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-         "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics"]
+         "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics"]
 
 
 @pytest.mark.parametrize("case", CASES)

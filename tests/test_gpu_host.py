@@ -40,13 +40,15 @@ def read_sim_out(path):
 
 @pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic_dynamics",
                                   "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-                                  "dist_harmonic_dynamics"])
+                                  "dist_harmonic_dynamics", "bosonic_factorial_harmonic_dynamics"])
 def test_driver_reproduces_reference_outputs(case, tmp_path):
     """One save interval (sfreq steps): frames 0 and 1 of every bead's
-    position, velocity and force file and rows 0 and 1 of simulation.out."""
+    position, velocity and force file and rows 0 and 1 of simulation.out
+    (the factorial case with the factorial exchange, as the reference's
+    harness builds it)."""
     out = tmp_path / "output"
     _, _, sfreq = H.load_config(ini(case))
-    H.run(ini(case), steps=sfreq, output_dir=str(out))
+    H.run(ini(case), steps=sfreq, output_dir=str(out), factorial="factorial" in case)
     fx = load_fixture(case)
     P = fx["positions"].shape[1]
     for b in range(P):

@@ -53,7 +53,7 @@ def test_observables_external_traps(ext):
 
 
 @pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic", "dist_harmonic",
-                                  "bosonic_quadratic_harmonic_nmthermostat_dynamics"])
+                                  "bosonic_quadratic_harmonic_nmthermostat_dynamics", "bosonic_factorial_harmonic"])
 def test_observables_reference_rows(case):
     """Steps on the GPU (reference noise) then its observables against the
     stored simulation.out rows (printed with 9 significant digits)."""
