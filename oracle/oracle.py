@@ -15,7 +15,7 @@ _DP = C.POINTER(C.c_double)
 
 POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
-THERMO = {"none": 0, "langevin": 1}
+THERMO = {"none": 0, "langevin": 1, "nose_hoover": 2, "nose_hoover_np": 3, "nose_hoover_np_dim": 4}
 EXCHANGE = {"quadratic": 0, "factorial": 1}
 
 
@@ -33,7 +33,7 @@ class Params(C.Structure):
         ("seed", C.c_uint),
         ("ext_strength", C.c_double), ("ext_location", C.c_double),
         ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
-        ("exchange_algo", C.c_int),
+        ("exchange_algo", C.c_int), ("nchains", C.c_int),
     ]
 
 
@@ -44,6 +44,7 @@ def make_params(**kw):
     p.ndim = 3
     p.thermostat = 1
     p.cutoff = -1.0
+    p.nchains = 4
     for k, v in kw.items():
         if k in ("ext_kind", "int_kind") and isinstance(v, str):
             v = POT[v]
@@ -240,7 +241,7 @@ class Sim:
             raise OracleError("forces returned %d" % rc)
 
     def observables(self):
-        out = np.zeros(10)
+        out = np.zeros(11)
         rc = lib().orc_sim_observables(self.h, ptr(out))
         if rc != 0:
             raise OracleError("observables returned %d" % rc)

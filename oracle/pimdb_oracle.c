@@ -883,10 +883,16 @@ struct orc_sim {
     orc_mt19937* mt;
     double c1, c2;                   /* Langevin coefficients */
     double *nm_freq;                 /* NM propagator, per bead */
+    /* Nose-Hoover chains, one thermostat object per bead (rank): chains of
+     * nchains components; per bead 1 chain (nose_hoover), N (np) or N*D (np_dim) */
+    int nh_groups, nh_ndof;          /* chains per bead, dof per chain */
+    double nh_Q1, nh_Qi, nh_required;
+    double *eta, *eta_dot, *eta_dot_dot;   /* [P][nh_groups][nchains] */
 };
 
 orc_sim* orc_sim_create(const orc_params* p) {
     if (p->natoms < 1 || p->nbeads < 1 || p->ndim < 1 || p->ndim > 8) return NULL;
+    if (p->thermostat >= ORC_THERMO_NOSE_HOOVER && p->nchains < 2) return NULL;   /* see nh_chain_step */
     orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
     s->p = *p;
     s->d = derive(p);
@@ -918,6 +924,28 @@ orc_sim* orc_sim_create(const orc_params* p) {
     /* normal_modes_propagator.cpp:13 */
     const double pi = 3.14159265358979323846;
     for (int b = 0; b < s->P; ++b) s->nm_freq[b] = 2 * s->d.omega_p * sin(b * pi / s->P);
+    /* nose_hoover.cpp:10-32, 152-163, 198-208 (hbar = 1) */
+    if (p->thermostat >= ORC_THERMO_NOSE_HOOVER) {
+        const int D = s->D, N = s->N, nc = p->nchains;
+        s->nh_Qi = 1.0 * 1.0 * s->d.beta / s->P;
+        if (p->thermostat == ORC_THERMO_NOSE_HOOVER) {
+            s->nh_groups = 1; s->nh_ndof = D * N;
+            s->nh_Q1 = D * N * s->nh_Qi;
+            s->nh_required = D * N / s->d.thermo_beta;
+        } else if (p->thermostat == ORC_THERMO_NOSE_HOOVER_NP) {
+            s->nh_groups = N; s->nh_ndof = D;
+            s->nh_Q1 = D * s->nh_Qi;
+            s->nh_required = D / s->d.thermo_beta;
+        } else {
+            s->nh_groups = N * D; s->nh_ndof = 1;
+            s->nh_Q1 = s->nh_Qi;
+            s->nh_required = 1 / s->d.thermo_beta;
+        }
+        const size_t n = (size_t)s->P * s->nh_groups * (nc > 0 ? nc : 1);
+        s->eta = (double*)calloc(n, sizeof(double));
+        s->eta_dot = (double*)calloc(n, sizeof(double));
+        s->eta_dot_dot = (double*)calloc(n, sizeof(double));
+    }
     return s;
 }
 
@@ -925,7 +953,8 @@ void orc_sim_destroy(orc_sim* s) {
     if (!s) return;
     free(s->x); free(s->pm); free(s->f); free(s->ext_forces); free(s->spring_forces);
     free(s->tmpf); free(s->pnm); free(s->C); free(s->CT); free(s->nm_a); free(s->nm_b);
-    free(s->nm_freq); free(s->mars); free(s->mt); free(s);
+    free(s->nm_freq); free(s->mars); free(s->mt);
+    free(s->eta); free(s->eta_dot); free(s->eta_dot_dot); free(s);
 }
 
 /* simulation.cpp:581-603; beads summed in order (see file header) */
@@ -989,7 +1018,127 @@ int orc_sim_forces(orc_sim* s) {
 }
 
 /* thermostat.cpp:15-19 + langevin.cpp:15-27 + thermostat_coupling.cpp */
+/* NoseHooverThermostat::singleChainStep, nose_hoover.cpp:96-150 (Tuckerman
+ * et al. 2006, LAMMPS), on one chain (e, ed, edd: its nchains components). */
+static double nh_chain_step(const orc_sim* s, double* e, double* ed, double* edd, double current_energy) {
+    const int nc = s->p.nchains;
+    const double dt2 = 0.5 * s->p.dt, dt4 = 0.25 * s->p.dt, dt8 = 0.125 * s->p.dt;
+    const double Q1 = s->nh_Q1, Qi = s->nh_Qi, tb = s->d.thermo_beta;
+    double exp_factor = 0.0;
+    edd[0] = (current_energy - s->nh_required) / Q1;
+    ed[nc - 1] += edd[nc - 1] * dt4;
+    for (int i = nc - 2; i >= 0; i--) {
+        exp_factor = exp(-dt8 * ed[i + 1]);
+        ed[i] *= exp_factor;
+        ed[i] += edd[i] * dt4;
+        ed[i] *= exp_factor;
+    }
+    double scale = exp(-dt2 * ed[0]);
+    for (int i = 0; i < nc; i++) e[i] += dt2 * ed[i];
+    edd[0] = (current_energy * scale * scale - s->nh_required) / Q1;
+    ed[0] *= exp_factor;
+    ed[0] += edd[0] * dt4;
+    ed[0] *= exp_factor;
+    double Q_former = Q1;
+    for (int i = 1; i < nc - 1; i++) {
+        exp_factor = exp(-dt8 * ed[i + 1]);
+        ed[i] *= exp_factor;
+        edd[i] = (Q_former * ed[i - 1] * ed[i - 1] - 1 / tb) / Qi;
+        ed[i] += edd[i] * dt4;
+        ed[i] *= exp_factor;
+        Q_former = Qi;
+    }
+    edd[nc - 1] = (Qi * ed[nc - 2] * ed[nc - 2] - 1 / tb) / Qi;
+    ed[nc - 1] += edd[nc - 1] * dt4;
+    return scale;
+}
+
+/* momentaUpdate of the three Nose-Hoover variants (nose_hoover.cpp:75-94,
+ * 165-184, 210-227) on bead b's momenta pb (Cartesian, or its normal mode) */
+static void nh_bead(orc_sim* s, int b, double* pb) {
+    const int N = s->N, D = s->D, nc = s->p.nchains;
+    const size_t base = (size_t)b * s->nh_groups * nc;
+    if (s->p.thermostat == ORC_THERMO_NOSE_HOOVER) {
+        double ce = 0.0;
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < D; ++a) ce += pb[i * D + a] * pb[i * D + a];
+        ce /= s->p.mass;
+        const double sc = nh_chain_step(s, s->eta + base, s->eta_dot + base, s->eta_dot_dot + base, ce);
+        for (int e = 0; e < N * D; ++e) pb[e] = pb[e] * sc;
+    } else if (s->p.thermostat == ORC_THERMO_NOSE_HOOVER_NP) {
+        for (int i = 0; i < N; ++i) {
+            double ce = 0.0;
+            for (int a = 0; a < D; ++a) ce += pb[i * D + a] * pb[i * D + a];
+            ce /= s->p.mass;
+            const size_t o = base + (size_t)i * nc;
+            const double sc = nh_chain_step(s, s->eta + o, s->eta_dot + o, s->eta_dot_dot + o, ce);
+            for (int a = 0; a < D; ++a) pb[i * D + a] = pb[i * D + a] * sc;
+        }
+    } else {
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < D; ++a) {
+                const double ce = pb[i * D + a] * pb[i * D + a] / s->p.mass;
+                const size_t o = base + ((size_t)i * D + a) * nc;
+                const double sc = nh_chain_step(s, s->eta + o, s->eta_dot + o, s->eta_dot_dot + o, ce);
+                pb[i * D + a] = pb[i * D + a] * sc;
+            }
+    }
+}
+
+/* getAdditionToH summed over the beads' thermostats (nose_hoover.cpp:41-66,
+ * 186-192, 229-237), the nh_energy column (classical.cpp:24-26) */
+static double nh_addition_to_h(const orc_sim* s) {
+    const int nc = s->p.nchains;
+    const double tb = s->d.thermo_beta;
+    double tot = 0.0;
+    for (int b = 0; b < s->P; ++b) {
+        double add = 0.0;
+        for (int g = 0; g < s->nh_groups; ++g) {
+            const size_t o = ((size_t)b * s->nh_groups + g) * nc;
+            double a = 0.5 * s->nh_Q1 * s->eta_dot[o] * s->eta_dot[o];
+            a += s->nh_ndof * s->eta[o] / tb;
+            for (int i = 1; i < nc; i++) {
+                a += 0.5 * s->nh_Qi * s->eta_dot[o + i] * s->eta_dot[o + i];
+                a += s->eta[o + i] / tb;
+            }
+            add += a;
+        }
+        tot += add;
+    }
+    return tot;
+}
+
 static void thermostat_step(orc_sim* s, const double* noise) {
+    if (s->p.thermostat >= ORC_THERMO_NOSE_HOOVER) {
+        const int N = s->N, D = s->D, P = s->P;
+        if (!s->p.nmthermostat) {
+            for (int b = 0; b < P; ++b) nh_bead(s, b, s->pm + s->nb * b);
+            return;
+        }
+        /* NM coupling (thermostat_coupling.cpp:28-47): mode b from the shared
+         * cartesian momenta, thermostatted, then all beads rebuilt */
+        for (int b = 0; b < P; ++b) {
+            const double* row = s->C + (size_t)b * P;
+            for (int i = 0; i < N; ++i)
+                for (int a = 0; a < D; ++a) {
+                    const size_t off = (size_t)i * D + a;
+                    double pnm = 0;
+                    for (int j = 0; j < P; ++j) pnm += row[j] * s->pm[s->nb * j + off];
+                    s->pnm[s->nb * b + off] = pnm;
+                }
+            nh_bead(s, b, s->pnm + s->nb * b);
+        }
+        for (int b = 0; b < P; ++b) {
+            const double* crow = s->CT + (size_t)b * P;
+            for (size_t off = 0; off < s->nb; ++off) {
+                double pc = 0;
+                for (int k = 0; k < P; ++k) pc += crow[k] * s->pnm[s->nb * k + off];
+                s->tmpf[s->nb * b + off] = pc;
+            }
+        }
+        memcpy(s->pm, s->tmpf, sizeof(double) * s->nb * P);
+        return;
+    }
     if (s->p.thermostat != ORC_THERMO_LANGEVIN) return;   /* "none": no-op */
     const int N = s->N, D = s->D, P = s->P;
     if (!s->p.nmthermostat) {
@@ -1203,6 +1352,7 @@ int orc_sim_observables(orc_sim* s, double* out) {
     out[0] = kinetic; out[1] = potential; out[2] = ext_pot_t; out[3] = int_pot_t;
     out[4] = virial_t; out[5] = temperature; out[6] = cl_kin; out[7] = cl_spring;
     out[8] = prob_dist; out[9] = prob_all;
+    out[10] = p->thermostat >= ORC_THERMO_NOSE_HOOVER ? nh_addition_to_h(s) : 0.0;
     free(E); free(V);
     return rc;
 }

@@ -31,7 +31,8 @@ enum { ORC_OK = 0, ORC_E_NONFINITE = 1, ORC_E_ARG = 2 };
 enum { ORC_POT_FREE = 0, ORC_POT_HARMONIC = 1, ORC_POT_DIPOLE = 2, ORC_POT_AZIZ = 3,
        ORC_POT_DOUBLE_WELL = 4, ORC_POT_COSINE = 5 };
 enum { ORC_PROP_CARTESIAN = 0, ORC_PROP_NORMAL_MODES = 1 };
-enum { ORC_THERMO_NONE = 0, ORC_THERMO_LANGEVIN = 1 };
+enum { ORC_THERMO_NONE = 0, ORC_THERMO_LANGEVIN = 1, ORC_THERMO_NOSE_HOOVER = 2, ORC_THERMO_NOSE_HOOVER_NP = 3,
+       ORC_THERMO_NOSE_HOOVER_NP_DIM = 4 };
 
 /* One ring-polymer system, atomic units (mirrors Params/Simulation fields,
  * src/simulation.cpp:21-59, src/params.cpp:14-247). */
@@ -52,6 +53,7 @@ typedef struct orc_params {
      * (amplitude: energy, phase: number; the wavelength is the box size) */
     double ext_strength, ext_location, ext_amplitude, ext_phase;
     int exchange_algo;  /* ORC_EXCHANGE_QUADRATIC or _FACTORIAL (FACTORIAL_BOSONIC_ALGORITHM) */
+    int nchains;        /* Nose-Hoover chain length (params.cpp:22, default 4) */
 } orc_params;
 
 enum { ORC_EXCHANGE_QUADRATIC = 0, ORC_EXCHANGE_FACTORIAL = 1 };
@@ -164,7 +166,7 @@ int orc_sim_forces(orc_sim* s);
 void orc_sim_zero_momentum(orc_sim* s);
 /* Observables summed over beads, atomic units:
  * [kinetic, potential, ext_pot, int_pot, virial, temperature(Hartree),
- *  cl_kinetic, cl_spring, prob_dist, prob_all]. */
+ *  cl_kinetic, cl_spring, prob_dist, prob_all, nh_energy]. */
 int orc_sim_observables(orc_sim* s, double* out);
 
 #ifdef __cplusplus

@@ -56,6 +56,7 @@ def case_params(case):
         # (FACTORIAL_BOSONIC_ALGORITHM); its harness runs the *factorial* cases
         # with that build (tests/main.py:286-290)
         exchange_algo=1 if "factorial" in case else 0,
+        nchains=int(sim.get("nchains", "4")),
     ), int(sim.get("sfreq", "1000"))
 
 

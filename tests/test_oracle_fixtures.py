@@ -12,7 +12,9 @@ from helpers import (ANGSTROM, FORCE_EV_A, KELVIN, VEL_A_PS, case_params, load_f
                      per_bead_err, rel_err)
 
 DYN = ["bosonic_quadratic_harmonic_dynamics", "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-       "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics"]
+       "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics",
+       "bosonic_quadratic_harmonic_nh_dynamics", "bosonic_quadratic_harmonic_nh_np_dynamics",
+       "bosonic_quadratic_harmonic_nh_np_dim_dynamics"]
 OUT = DYN + ["bosonic_quadratic_harmonic", "dist_harmonic", "bosonic_factorial_harmonic"]
 
 
@@ -60,7 +62,7 @@ def test_trajectory_matches_fixture(case):
 
 
 NAMES = ["kinetic", "potential", "ext_pot", "int_pot", "virial", "temperature", "cl_kinetic",
-         "cl_spring", "prob_dist", "prob_all"]
+         "cl_spring", "prob_dist", "prob_all", "nh_energy"]
 
 
 @pytest.mark.parametrize("case", OUT)
