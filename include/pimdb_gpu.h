@@ -41,7 +41,10 @@ enum pimdb_potential {
     PIMDB_POT_DOUBLE_WELL = 4, PIMDB_POT_COSINE = 5
 };
 enum pimdb_propagator { PIMDB_PROP_CARTESIAN = 0, PIMDB_PROP_NORMAL_MODES = 1 };
-enum pimdb_thermostat { PIMDB_THERMO_NONE = 0, PIMDB_THERMO_LANGEVIN = 1 };
+/* Nose-Hoover chains (nose_hoover.cpp): one chain per bead, per particle of
+ * each bead (np) or per degree of freedom of each bead (np_dim). */
+enum pimdb_thermostat { PIMDB_THERMO_NONE = 0, PIMDB_THERMO_LANGEVIN = 1, PIMDB_THERMO_NOSE_HOOVER = 2,
+                        PIMDB_THERMO_NOSE_HOOVER_NP = 3, PIMDB_THERMO_NOSE_HOOVER_NP_DIM = 4 };
 
 /* One ring-polymer system.  Replaces the Params maps and the Simulation
  * fields read from them (src/params.cpp:14-247, src/simulation.cpp:21-93).
@@ -69,6 +72,9 @@ typedef struct pimdb_params {
      * reference chooses between them at build time
      * (FACTORIAL_BOSONIC_ALGORITHM, simulation.cpp:694-698) */
     int exchange_algo;
+    /* Nose-Hoover chain length (the [simulation] key nchains, default 4,
+     * params.cpp:22-25); used by the Nose-Hoover thermostats, >= 2 */
+    int nchains;
 } pimdb_params;
 
 enum pimdb_exchange_algo { PIMDB_EXCHANGE_QUADRATIC = 0, PIMDB_EXCHANGE_FACTORIAL = 1 };
@@ -119,9 +125,12 @@ int pimdb_step(pimdb_ctx* ctx, int nsteps, const double* host_noise);
 
 /* Observables summed over beads, atomic units (EnergyObservable,
  * ClassicalObservable, BosonicObservable; src/observables/{energy,classical,
- * bosonic}.cpp): out[10] = {kinetic, potential, ext_pot, int_pot, virial,
- * temperature, cl_kinetic, cl_spring, prob_dist, prob_all}.  ext_pot/int_pot
- * are 0 unless both potentials are non-free, as in energy.cpp:96-102. */
+ * bosonic}.cpp): out[11] = {kinetic, potential, ext_pot, int_pot, virial,
+ * temperature, cl_kinetic, cl_spring, prob_dist, prob_all, nh_energy}.
+ * ext_pot/int_pot are 0 unless both potentials are non-free, as in
+ * energy.cpp:96-102; nh_energy (the Nose-Hoover thermostats'
+ * getAdditionToH summed over beads, classical.cpp:24-26) is 0 for the other
+ * thermostats. */
 int pimdb_observables(pimdb_ctx* ctx, double* out);
 
 /* Exchange estimators from the current positions of beads 1 and P:

@@ -80,7 +80,11 @@ inline bool validate_params(const pimdb_params& p, std::string* why) {
     if (p.propagator != PIMDB_PROP_CARTESIAN && p.propagator != PIMDB_PROP_NORMAL_MODES) return bad("unknown propagator");
     if (p.bosonic && p.propagator == PIMDB_PROP_NORMAL_MODES)
         return bad("Normal modes propogation is currently not available for bosons!");
-    if (p.thermostat != PIMDB_THERMO_NONE && p.thermostat != PIMDB_THERMO_LANGEVIN) return bad("unknown thermostat");
+    if (p.thermostat < PIMDB_THERMO_NONE || p.thermostat > PIMDB_THERMO_NOSE_HOOVER_NP_DIM) return bad("unknown thermostat");
+    // nose_hoover.cpp:141-143 reads eta_dot[nchains - 2] of the chain, outside
+    // it for nchains = 1, so chains of one component are not supported
+    if (p.thermostat >= PIMDB_THERMO_NOSE_HOOVER && p.nchains < 2)
+        return bad("Nose-Hoover chains need nchains >= 2");
     if (p.nmthermostat && p.thermostat == PIMDB_THERMO_NONE) return bad("nmthermostat cannot be used in nve ensemble!");
     if (p.nbeads > 2048) return bad("nbeads above 2048 is not supported");
     if (p.exchange_algo != PIMDB_EXCHANGE_QUADRATIC && p.exchange_algo != PIMDB_EXCHANGE_FACTORIAL)

@@ -34,6 +34,12 @@ struct MomentaArgs {
     int ncom_in;                // number of partial rows [ncom_in][D]
     double* com_out;            // partial sums of the result, [gridDim.x][D], or null
     const double* kick_f;       // forces for a half kick first, or null
+    // 0: the whole O-step.  Normal-mode coupling with a thermostat that acts
+    // on all modes at once (Nose-Hoover): 1 writes the modes of the staged
+    // momenta to `modes` [k][a][i] and stops; 2 reads the (thermostatted)
+    // modes back, transforms them to p and writes the partial sums.
+    int stage;
+    double* modes;
 };
 
 // Centre-of-mass momentum per axis from the partial sums [npart][D], reduced
@@ -90,7 +96,32 @@ static __global__ void __launch_bounds__(256) k_momenta(MomentaArgs m, double* _
         }
         pc[e] = v;
     }
-    if (m.thermostat == 1) {
+    if (m.stage == 1) {
+        __syncthreads();
+        for (int e = tid; e < P * TC; e += blockDim.x) {
+            const int k = e / TC, c = e % TC;
+            if (c < ncol) {
+                double s = 0.0;
+                for (int j = 0; j < P; ++j) s += m.Ct[(size_t)j * P + k] * pc[j * TC + c];
+                m.modes[((size_t)k * D + a) * N + i0 + c] = s;
+            }
+        }
+        return;
+    }
+    if (m.stage == 2) {
+        __syncthreads();
+        for (int e = tid; e < P * TC; e += blockDim.x) {
+            const int k = e / TC, c = e % TC;
+            pn[e] = c < ncol ? m.modes[((size_t)k * D + a) * N + i0 + c] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < P * TC; e += blockDim.x) {
+            const int b = e / TC, c = e % TC;
+            double s = 0.0;
+            for (int k = 0; k < P; ++k) s += m.CTt[(size_t)k * P + b] * pn[k * TC + c];
+            pc[e] = s;
+        }
+    } else if (m.thermostat == 1) {
         const int half = (P + 1) / 2;
         if (!m.nm) {
             for (int e = tid; e < half * TC; e += blockDim.x) {

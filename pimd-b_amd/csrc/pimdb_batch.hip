@@ -174,6 +174,10 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             set_create_error("batched replicas use the Feldman-Hirshberg exchange");
             return PIMDB_E_ARG;
         }
+        if (reps[r].thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
+            set_create_error("batched replicas use the Langevin thermostat or none");
+            return PIMDB_E_ARG;
+        }
         if (reps[r].propagator != PIMDB_PROP_CARTESIAN) {
             set_create_error("batched replicas use the cartesian propagator");
             return PIMDB_E_ARG;

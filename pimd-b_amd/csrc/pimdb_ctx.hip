@@ -23,6 +23,7 @@
 #include "exchange.hpp"
 #include "factorial.hpp"
 #include "forces.hpp"
+#include "nose_hoover.hpp"
 #include "integrator.hpp"
 #include "observables.hpp"
 
@@ -44,6 +45,9 @@ struct pimdb_ctx {
     int* status = nullptr;
     unsigned long long* ctr = nullptr;
     int tc = 1, ntiles = 1;
+    int ncom = 1;                  // rows of the partial momentum sums (com0 / com1)
+    NhArgs nh{};                   // Nose-Hoover chains (thermostat >= 2)
+    double* modes = nullptr;       // normal modes of p, [k][a][i] (Nose-Hoover with nmthermostat)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<double> stage;
@@ -190,23 +194,62 @@ static size_t momenta_lds(pimdb_ctx* ctx) {
 }
 
 // One OBABO step; noise0/noise1: device noise of the two O-steps or null.
+// One Nose-Hoover O-step: the pending zeroMomentum (com_in), the half kick
+// (kick_f), the chains, the partial sums of the result (com_out).  Under
+// normal-mode coupling the chains act on the modes: transform (k_momenta
+// stage 1), chains, back-transform with the partial sums (stage 2).
+static void enqueue_nh_ostep(pimdb_ctx* ctx, const double* com_in, const double* kick_f, double* com_out) {
+    NhArgs h = ctx->nh;
+    h.half_dt = 0.5 * ctx->prm.dt;
+    if (!ctx->prm.nmthermostat) {
+        h.com_in = com_in;
+        h.ncom_in = ctx->ncom;
+        h.kick_f = kick_f;
+        h.com_out = com_out;
+        hipLaunchKernelGGL(k_nh_ostep, dim3(ctx->P), dim3(256), 0, ctx->s0, h, ctx->p);
+        return;
+    }
+    dim3 tgrid(ctx->ntiles, ctx->D);
+    MomentaArgs m = momenta_args(ctx, 0, nullptr);
+    m.thermostat = 0;
+    m.stage = 1;
+    m.modes = ctx->modes;
+    m.com_in = com_in;
+    m.ncom_in = ctx->ncom;
+    m.kick_f = kick_f;
+    hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m, ctx->p);
+    h.com_in = nullptr;
+    h.kick_f = nullptr;
+    h.com_out = nullptr;
+    hipLaunchKernelGGL(k_nh_ostep, dim3(ctx->P), dim3(256), 0, ctx->s0, h, ctx->modes);
+    m.stage = 2;
+    m.com_in = nullptr;
+    m.kick_f = nullptr;
+    m.com_out = com_out;
+    hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m, ctx->p);
+}
+
 static int enqueue_step(pimdb_ctx* ctx, const double* noise0, const double* noise1) {
     const bool fixcom = ctx->prm.fixcom != 0;
     const int N = ctx->N, P = ctx->P, D = ctx->D;
     dim3 tgrid(ctx->ntiles, D);
     // K1: deferred zeroMomentum + O-step
-    MomentaArgs m1 = momenta_args(ctx, 0, noise0);
-    m1.com_in = fixcom ? ctx->com1 : nullptr;
-    m1.ncom_in = ctx->ntiles;
-    m1.com_out = fixcom ? ctx->com0 : nullptr;
-    hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m1, ctx->p);
+    if (ctx->prm.thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
+        enqueue_nh_ostep(ctx, fixcom ? ctx->com1 : nullptr, nullptr, fixcom ? ctx->com0 : nullptr);
+    } else {
+        MomentaArgs m1 = momenta_args(ctx, 0, noise0);
+        m1.com_in = fixcom ? ctx->com1 : nullptr;
+        m1.ncom_in = ctx->ncom;
+        m1.com_out = fixcom ? ctx->com0 : nullptr;
+        hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m1, ctx->p);
+    }
     // K2: zeroMomentum + B + A   (or the normal-mode propagator)
     if (ctx->prm.propagator == PIMDB_PROP_NORMAL_MODES) {
         NmPropArgs a{};
         a.N = N; a.P = P; a.D = D; a.tc = ctx->tc;
         a.half_dt = 0.5 * ctx->prm.dt; a.dt = ctx->prm.dt; a.mass = ctx->prm.mass;
         a.Ct = ctx->C; a.CTt = ctx->CT; a.rot = ctx->rot;
-        a.com_in = fixcom ? ctx->com0 : nullptr; a.ncom_in = ctx->ntiles;
+        a.com_in = fixcom ? ctx->com0 : nullptr; a.ncom_in = ctx->ncom;
         a.fext = ctx->fext; a.step_ctr = ctx->ctr;
         hipLaunchKernelGGL(k_nm_propagate, tgrid, dim3(256), sizeof(double) * 4 * (size_t)P * ctx->tc, ctx->s0, a,
                            ctx->x, ctx->p);
@@ -214,7 +257,7 @@ static int enqueue_step(pimdb_ctx* ctx, const double* noise0, const double* nois
         DriftArgs d{};
         d.N = N; d.P = P; d.D = D;
         d.half_dt = 0.5 * ctx->prm.dt; d.dt = ctx->prm.dt; d.mass = ctx->prm.mass;
-        d.com_in = fixcom ? ctx->com0 : nullptr; d.ncom_in = ctx->ntiles;
+        d.com_in = fixcom ? ctx->com0 : nullptr; d.ncom_in = ctx->ncom;
         d.step_ctr = ctx->ctr;
         hipLaunchKernelGGL(k_kick_drift, dim3((unsigned)((ctx->nbd + 255) / 256)), dim3(256), 0, ctx->s0, d, ctx->x,
                            ctx->p, ctx->f);
@@ -223,10 +266,15 @@ static int enqueue_step(pimdb_ctx* ctx, const double* noise0, const double* nois
     int rc = enqueue_forces(ctx);
     if (rc) return rc;
     // K4: B + O-step + partial COM sums
-    MomentaArgs m2 = momenta_args(ctx, 1, noise1);
-    m2.kick_f = ctx->prm.propagator == PIMDB_PROP_NORMAL_MODES ? ctx->fext : ctx->f;
-    m2.com_out = fixcom ? ctx->com1 : nullptr;
-    hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m2, ctx->p);
+    const double* kick = ctx->prm.propagator == PIMDB_PROP_NORMAL_MODES ? ctx->fext : ctx->f;
+    if (ctx->prm.thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
+        enqueue_nh_ostep(ctx, nullptr, kick, fixcom ? ctx->com1 : nullptr);
+    } else {
+        MomentaArgs m2 = momenta_args(ctx, 1, noise1);
+        m2.kick_f = kick;
+        m2.com_out = fixcom ? ctx->com1 : nullptr;
+        hipLaunchKernelGGL(k_momenta, tgrid, dim3(256), momenta_lds(ctx), ctx->s0, m2, ctx->p);
+    }
     CK(hipGetLastError());
     return PIMDB_OK;
 }
@@ -235,8 +283,8 @@ static int enqueue_step(pimdb_ctx* ctx, const double* noise0, const double* nois
 static int finish_steps(pimdb_ctx* ctx) {
     if (ctx->prm.fixcom) {
         hipLaunchKernelGGL(k_com_subtract, dim3((unsigned)((ctx->nbd + 255) / 256)), dim3(256), 0, ctx->s0, ctx->N,
-                           ctx->P, ctx->D, ctx->com1, ctx->ntiles, ctx->p);
-        CK(hipMemsetAsync(ctx->com1, 0, sizeof(double) * (size_t)ctx->ntiles * ctx->D, ctx->s0));
+                           ctx->P, ctx->D, ctx->com1, ctx->ncom, ctx->p);
+        CK(hipMemsetAsync(ctx->com1, 0, sizeof(double) * (size_t)ctx->ncom * ctx->D, ctx->s0));
     }
     CK(hipGetLastError());
     return PIMDB_OK;
@@ -311,9 +359,30 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     alloc(&ctx->C, sizeof(double) * (size_t)P * P);
     alloc(&ctx->CT, sizeof(double) * (size_t)P * P);
     alloc(&ctx->rot, sizeof(double) * 4 * (size_t)P);
-    alloc(&ctx->com0, sizeof(double) * (size_t)ctx->ntiles * D);
-    alloc(&ctx->com1, sizeof(double) * (size_t)ctx->ntiles * D);
-    alloc(&ctx->obs, sizeof(double) * (2 * (size_t)P + 3 * (size_t)P * ((N + FORCE_TILE - 1) / FORCE_TILE) + 4));
+    // partial momentum sums: one row per k_momenta tile, or per bead when the
+    // Cartesian Nose-Hoover O-step (one workgroup per bead) produces them
+    const bool nh = prm->thermostat >= PIMDB_THERMO_NOSE_HOOVER;
+    ctx->ncom = (nh && !prm->nmthermostat) ? P : ctx->ntiles;
+    alloc(&ctx->com0, sizeof(double) * (size_t)ctx->ncom * D);
+    alloc(&ctx->com1, sizeof(double) * (size_t)ctx->ncom * D);
+    if (nh) {
+        NhArgs& h = ctx->nh;
+        h.N = N; h.P = P; h.D = D; h.nc = prm->nchains;
+        h.kind = prm->thermostat;
+        h.groups = prm->thermostat == PIMDB_THERMO_NOSE_HOOVER ? 1 : prm->thermostat == PIMDB_THERMO_NOSE_HOOVER_NP ? N : N * D;
+        const int ndof = prm->thermostat == PIMDB_THERMO_NOSE_HOOVER ? D * N : prm->thermostat == PIMDB_THERMO_NOSE_HOOVER_NP ? D : 1;
+        h.mass = prm->mass; h.dt = prm->dt;
+        h.thermo_beta = ctx->dv.thermo_beta;   // beta / P
+        h.Qi = 1.0 * 1.0 * ctx->dv.beta / P;    // hbar^2 beta / P, nose_hoover.cpp:17
+        h.Q1 = ndof * h.Qi;
+        h.required = ndof / h.thermo_beta;
+        const size_t nch = (size_t)P * h.groups * h.nc;
+        alloc(&h.eta, sizeof(double) * nch);
+        alloc(&h.eta_dot, sizeof(double) * nch);
+        alloc(&h.eta_dot_dot, sizeof(double) * nch);
+        if (prm->nmthermostat) alloc(&ctx->modes, sizeof(double) * ctx->nbd);
+    }
+    alloc(&ctx->obs, sizeof(double) * (2 * (size_t)P + 3 * (size_t)P * ((N + FORCE_TILE - 1) / FORCE_TILE) + 5));
     alloc(&ctx->status, sizeof(int));
     alloc(&ctx->ctr, sizeof(unsigned long long));
     if (!rc && (hipStreamCreateWithFlags(&ctx->s0, hipStreamNonBlocking) != hipSuccess ||
@@ -357,7 +426,8 @@ void pimdb_ctx_destroy(pimdb_ctx* ctx) {
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     double* bufs[] = {ctx->x, ctx->p, ctx->f, ctx->fext, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->prob, ctx->fex,
-                      ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs, ctx->Shi, ctx->Slo};
+                      ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs, ctx->Shi, ctx->Slo,
+                      ctx->nh.eta, ctx->nh.eta_dot, ctx->nh.eta_dot_dot, ctx->modes};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->status) (void)hipFree(ctx->status);
@@ -544,15 +614,21 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
         hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
                            ctx->dv.bt, ctx->E, ctx->V, prim);
     }
+    double* nh_out = prim + 3;   // [1]
+    if (ctx->prm.thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
+        const int ndof = ctx->nh.kind == PIMDB_THERMO_NOSE_HOOVER ? D * N : ctx->nh.kind == PIMDB_THERMO_NOSE_HOOVER_NP ? D : 1;
+        hipLaunchKernelGGL(k_nh_energy, dim3(1), dim3(256), 0, ctx->s0, ctx->nh, ndof, nh_out);
+    }
     CK(hipGetLastError());
     int rc = check_status(ctx);
     if (rc) return rc;
-    std::vector<double> h(2 * (size_t)P + 3 * (size_t)P * nt + 3);
+    std::vector<double> h(2 * (size_t)P + 3 * (size_t)P * nt + 4);
     CK(hipMemcpy(h.data(), ctx->obs, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     double VN = 0.0;
     if (ctx->dv.bosonic) CK(hipMemcpy(&VN, ctx->V + N, sizeof(double), hipMemcpyDeviceToHost));
     combine_observables(ctx->prm, ctx->dv, h.data(), h.data() + 2 * (size_t)P, nt, h.data() + 2 * (size_t)P + 3 * (size_t)P * nt,
                         VN, out);
+    out[10] = ctx->prm.thermostat >= PIMDB_THERMO_NOSE_HOOVER ? h[2 * (size_t)P + 3 * (size_t)P * nt + 3] : 0.0;
     return PIMDB_OK;
 }
 

@@ -177,8 +177,14 @@ Simulation::Simulation(Params& params, const RunOptions& options) : opts_(option
     else propagator_ = std::make_unique<VelocityVerletPropagator>();
     if (thermostat_type == "langevin") thermostat_ = std::make_unique<LangevinThermostat>(nmthermostat, gamma);
     else if (thermostat_type == "none") thermostat_ = std::make_unique<Thermostat>(nmthermostat);
-    else throw std::invalid_argument("The specified thermostat (" + thermostat_type +
-                                     ") is not available on the GPU path");
+    else {
+        // simulation.cpp:670-681
+        const int nchains = params.get<int>(params.sim, "nchains");
+        const int kind = thermostat_type == "nose_hoover"      ? PIMDB_THERMO_NOSE_HOOVER
+                         : thermostat_type == "nose_hoover_np" ? PIMDB_THERMO_NOSE_HOOVER_NP
+                                                               : PIMDB_THERMO_NOSE_HOOVER_NP_DIM;
+        thermostat_ = std::make_unique<NoseHooverThermostat>(nmthermostat, kind, nchains);
+    }
 
     external_potential_name = params.get<std::string>(params.external_pot, "name");
     interaction_potential_name = params.get<std::string>(params.interaction_pot, "name");
@@ -439,6 +445,8 @@ void Simulation::open_outputs(const Params& params) {
         quantities_.push_back({"temperature", "temperature", "kelvin", 5});
         quantities_.push_back({"cl_kinetic", "energy", u, 6});
         quantities_.push_back({"cl_spring", "energy", u, 7});
+        // classical.cpp:13-15, 24-26
+        if (thermostat_type.rfind("nose_hoover", 0) == 0) quantities_.push_back({"nh_energy", "energy", u, 10});
     }
     if (bosonic && params.observables.at("bosonic") != "off") {
         quantities_.push_back({"prob_dist", "number", "", 8});
@@ -489,7 +497,7 @@ void Simulation::log_observables(long step) {
         obs_file_ << centred(sci((double)step, 8), 16) << '\n';
         return;
     }
-    double out[10];
+    double out[11];
     check_ctx(pimdb_observables(ctx_, out), ctx_);
     obs_file_ << centred(sci((double)step, 8), 16);
     for (const Quantity& q : quantities_) obs_file_ << " " << centred(sci(to_user(q.family, q.unit, out[q.index]), 8), 16);

@@ -108,6 +108,19 @@ public:
 private:
     double gamma_;
 };
+// NoseHooverThermostat and its per-particle / per-degree-of-freedom variants
+// (nose_hoover.cpp); the chains live in the GPU context.
+class NoseHooverThermostat : public Thermostat {
+public:
+    NoseHooverThermostat(bool nmthermostat, int kind, int nchains) : Thermostat(nmthermostat), kind_(kind), nchains_(nchains) {}
+    void configure(pimdb_params& p) const override {
+        p.thermostat = kind_;
+        p.nmthermostat = nm_;
+        p.nchains = nchains_;
+    }
+private:
+    int kind_, nchains_;
+};
 
 // ---- the exchange (bosonic_exchange_base.h:16-54) on a GPU context ----
 class GpuBosonicExchange {

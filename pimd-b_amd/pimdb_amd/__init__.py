@@ -16,7 +16,7 @@ _DP = C.POINTER(C.c_double)
 OK, E_NONFINITE, E_ARG, E_HIP, E_STATE = 0, 1, 2, 3, 4
 POT = {"free": 0, "harmonic": 1, "dipole": 2, "aziz": 3, "double_well": 4, "cosine": 5}
 PROP = {"cartesian": 0, "normal_modes": 1}
-THERMO = {"none": 0, "langevin": 1}
+THERMO = {"none": 0, "langevin": 1, "nose_hoover": 2, "nose_hoover_np": 3, "nose_hoover_np_dim": 4}
 EXCHANGE = {"quadratic": 0, "factorial": 1}
 
 
@@ -44,7 +44,7 @@ class Params(C.Structure):
         ("seed", C.c_uint),
         ("ext_strength", C.c_double), ("ext_location", C.c_double),
         ("ext_amplitude", C.c_double), ("ext_phase", C.c_double),
-        ("exchange_algo", C.c_int),
+        ("exchange_algo", C.c_int), ("nchains", C.c_int),
     ]
 
 
@@ -53,6 +53,7 @@ def make_params(**kw):
     p.ndim = 3
     p.thermostat = 1
     p.cutoff = -1.0
+    p.nchains = 4
     for k, v in kw.items():
         if k in ("ext_kind", "int_kind") and isinstance(v, str):
             v = POT[v]
@@ -201,7 +202,7 @@ class Context:
         self._chk(lib().pimdb_step(self.h, int(nsteps), _p(noise)))
 
     def observables(self):
-        out = np.zeros(10)
+        out = np.zeros(11)
         self._chk(lib().pimdb_observables(self.h, _p(out)))
         return out
 

@@ -36,7 +36,9 @@ def run_pair(kw, nsteps, x=None, p=None, init_random=False, forces_first=False):
 
 
 CASES = ["bosonic_quadratic_harmonic_dynamics", "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-         "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics"]
+         "dist_harmonic_dynamics", "dist_harmonic_nm_propagation_dynamics", "bosonic_factorial_harmonic_dynamics",
+         "bosonic_quadratic_harmonic_nh_dynamics", "bosonic_quadratic_harmonic_nh_np_dynamics",
+         "bosonic_quadratic_harmonic_nh_np_dim_dynamics"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -112,3 +114,18 @@ def test_device_noise_thermalizes():
         temps.append(ctx.observables()[5])   # quantum temperature estimator (classical / P)
     ctx.close()
     assert abs(np.mean(temps) / T - 1.0) < 0.05
+
+
+@pytest.mark.parametrize("kind", ["nose_hoover", "nose_hoover_np", "nose_hoover_np_dim"])
+@pytest.mark.parametrize("fixcom", [0, 1])
+def test_nose_hoover_normal_mode_coupling_follows_oracle(kind, fixcom):
+    """Nose-Hoover chains on the normal modes (nmthermostat; no reference
+    fixture covers it): 200 steps against the oracle, bosonic and
+    distinguishable rings, with and without zeroMomentum."""
+    for bosonic in (1, 0):
+        kw, _ = case_params("bosonic_quadratic_harmonic_nh_dynamics")
+        kw.update(thermostat=kind, nmthermostat=1, fixcom=fixcom, bosonic=bosonic, nchains=3, nbeads=6)
+        (xg, pg, fg), (xo, po, fo) = run_pair(kw, 200, init_random=True)
+        assert rel_err(xg, xo) < 1e-10, bosonic
+        assert rel_err(pg, po) < 1e-10, bosonic
+        assert per_bead_err(fg, fo) < 1e-10, bosonic

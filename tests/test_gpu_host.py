@@ -40,7 +40,8 @@ def read_sim_out(path):
 
 @pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic_dynamics",
                                   "bosonic_quadratic_harmonic_nmthermostat_dynamics",
-                                  "dist_harmonic_dynamics", "bosonic_factorial_harmonic_dynamics"])
+                                  "dist_harmonic_dynamics", "bosonic_factorial_harmonic_dynamics",
+                                  "bosonic_quadratic_harmonic_nh_np_dynamics"])
 def test_driver_reproduces_reference_outputs(case, tmp_path):
     """One save interval (sfreq steps): frames 0 and 1 of every bead's
     position, velocity and force file and rows 0 and 1 of simulation.out

@@ -11,7 +11,7 @@ from helpers import KELVIN, aziz_config, case_params, load_fixture
 pytestmark = pytest.mark.gpu
 
 NAMES = ["kinetic", "potential", "ext_pot", "int_pot", "virial", "temperature", "cl_kinetic",
-         "cl_spring", "prob_dist", "prob_all"]
+         "cl_spring", "prob_dist", "prob_all", "nh_energy"]
 
 
 def compare(kw, x, p, rtol=1e-9):
@@ -53,7 +53,9 @@ def test_observables_external_traps(ext):
 
 
 @pytest.mark.parametrize("case", ["bosonic_quadratic_harmonic", "dist_harmonic",
-                                  "bosonic_quadratic_harmonic_nmthermostat_dynamics", "bosonic_factorial_harmonic"])
+                                  "bosonic_quadratic_harmonic_nmthermostat_dynamics", "bosonic_factorial_harmonic",
+                                  "bosonic_quadratic_harmonic_nh_dynamics", "bosonic_quadratic_harmonic_nh_np_dynamics",
+                                  "bosonic_quadratic_harmonic_nh_np_dim_dynamics"])
 def test_observables_reference_rows(case):
     """Steps on the GPU (reference noise) then its observables against the
     stored simulation.out rows (printed with 9 significant digits)."""
