@@ -21,15 +21,18 @@
 // (quadratic_bosonic_exchange.cpp:73-128).  Instead of one log-sum-exp per n
 // (a reduction, an exp and a log on the sequential path), targets are taken in
 // blocks of REC_B and each block runs
-//   (1) a min-plus pass  R_n = min_q (w(n,q) + Uhat_q)   [sequential part: add+min]
-//   (2) a linear solve   z_n = sum_q exp(-(w(n,q) + Uhat_q - R_n)) zhat_q
+//   (1) one pass over the sources of earlier blocks, giving per target
+//       m_n = min_q (w(n,q) + U_q) and s_n = sum_q exp(-(w(n,q) + U_q - m_n))
+//       (an online log-sum-exp: the partial sum is rescaled when m drops)
+//   (2) a min-plus pass  R_n = min(m_n, min_q (w(n,q) + R_q))  [sequential part: add+min]
+//   (3) a linear solve   z_n = s_n exp(-(m_n - R_n)) + sum_q exp(-(w(n,q) + R_q - R_n)) z_q
 //                                                         [sequential part: one FMA]
-//   (3) U_n = R_n - ln z_n
-// where Uhat_q = U_q, zhat_q = 1 for sources of earlier blocks and
-// Uhat_q = R_q, zhat_q = z_q inside the block.  Every exponent is <= 0 and the
-// minimising term contributes exactly zhat >= 1, so 1 <= z_n <= n 2^REC_B: no
-// overflow or underflow for any N.  All exponentials are off the sequential
-// path.  The result equals the reference's shifted log-sum-exp up to rounding.
+//   (4) U_n = R_n - ln z_n
+// where q in (2) and (3) runs over the block's own earlier targets.  Every
+// exponent is <= 0 and the minimising term contributes at least 1, so
+// 1 <= z_n <= n 2^REC_B: no overflow or underflow for any N.  All
+// exponentials are off the sequential path.  The result equals the
+// reference's shifted log-sum-exp up to rounding.
 #pragma once
 
 #include "device_common.hpp"
@@ -196,7 +199,8 @@ struct RecShared {
     double* U;         // [N+1]
     double* lg;        // [N+1]  ln(i)
     double* R;         // [REC_B] shifts of the block's targets
-    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min / sum
+    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min, then the earlier-source sums
+    double* psum;      // [REC_MAXWAVES][REC_B] per-wave partial sums scaled to that wave's min
     double* a;         // [REC_B*REC_B] in-block weights, then coefficients, a[j*REC_B + t]
     double* x1;        // [D][N] bead 1
     double* xP;        // [D][N] bead P
@@ -205,7 +209,7 @@ struct RecShared {
 };
 
 __host__ __device__ __forceinline__ size_t rec_shared_doubles(int N, int D) {
-    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B +
+    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B +
            (size_t)(2 * D + 2) * N;
 }
 
@@ -216,6 +220,7 @@ __device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
     s.lg = p; p += N + 1;
     s.R = p; p += REC_B;
     s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
+    s.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
     s.a = p; p += (size_t)REC_B * REC_B;
     s.x1 = p; p += (size_t)D * N;
     s.xP = p; p += (size_t)D * N;
@@ -233,7 +238,7 @@ struct RecDual {
     double* tmp;
 };
 __host__ __device__ inline size_t rec_dir_doubles(int N) {
-    return 2 * (size_t)(N + 1) + REC_B + (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
+    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
 }
 __host__ __device__ inline size_t rec_dual_doubles(int N, int D) {
     return (size_t)(2 * D + 2) * N + 2 * rec_dir_doubles(N);
@@ -253,6 +258,7 @@ __device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D) {
         sq.lg = p; p += N + 1;
         sq.R = p; p += REC_B;
         sq.part = p; p += (size_t)REC_MAXWAVES * REC_B;
+        sq.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
         sq.a = p; p += (size_t)REC_B * REC_B;
         sq.x1 = x1;
         sq.xP = xP;
@@ -365,24 +371,38 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         // (0) in-block weights w(n0+t, n0+j), j < t, staged as a[j][t]
         for (int j = wave; j < REC_B; j += nthr / WAVE)
             sh.a[j * REC_B + lane] = (lane < nb && j < lane) ? rec_w<DIR, D>(sh, r, bt, N, tg, n0 + j) : 0.0;
-        // (1a) per-wave partial min over this wave's share of earlier sources
+        // (1) one pass over this wave's share of earlier sources: the lane's
+        // running minimum m of y_q = w(n,q) + U_q and s = sum_q exp(-(y_q - m)),
+        // rescaled whenever a chunk lowers m
         const int chunk = (n0 + nwave - 1) / nwave;
         const int qlo = min(n0, wave * chunk), qhi = min(n0, qlo + chunk);
         const bool active = wave < nwave && lane < nb;
         if (wave < nwave) {
-            double m = INFINITY;
+            double m = INFINITY, s = 0.0;
             if (active) {
                 int q = qlo;
                 for (; q + UNROLL <= qhi; q += UNROLL) {
-                    double w[UNROLL];
+                    double y[UNROLL];
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) w[u] = rec_w<DIR, D>(sh, r, bt, N, tg, q + u);
+                    for (int u = 0; u < UNROLL; ++u) y[u] = rec_w<DIR, D>(sh, r, bt, N, tg, q + u) + sh.U[q + u];
+                    double cm = y[0];
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) m = fmin(m, w[u] + sh.U[q + u]);
+                    for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
+                    const double mn = fmin(m, cm);
+                    s *= exp_minimax(mn - m);   // m = inf (first chunk): s = 0 stays 0
+                    m = mn;
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) s += exp_minimax(mn - y[u]);
                 }
-                for (; q < qhi; ++q) m = fmin(m, rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q]);
+                for (; q < qhi; ++q) {
+                    const double y = rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q];
+                    const double mn = fmin(m, y);
+                    s = fma(s, exp_minimax(mn - m), exp_minimax(mn - y));
+                    m = mn;
+                }
             }
             sh.part[wave * REC_B + lane] = m;
+            sh.psum[wave * REC_B + lane] = s;
         }
         __syncthreads();
         // (1b) in-block min-plus on wave 0
@@ -401,34 +421,30 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             sh.R[t] = rr;
         }
         __syncthreads();
-        // (2a) per-wave partial sums over earlier sources
-        if (wave < nwave) {
-            double s = 0.0;
-            if (active) {
-                const double Rn = sh.R[lane];
-                int q = qlo;
-                for (; q + UNROLL <= qhi; q += UNROLL) {
-                    double w[UNROLL];
-#pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) w[u] = rec_w<DIR, D>(sh, r, bt, N, tg, q + u);
-#pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) s += exp(-(w[u] + sh.U[q + u] - Rn));
+        // (2a) the earlier sources' sum relative to R_t, sum_w s_w exp(-(m_w - R_t)),
+        // added in wave order into part[0][t]
+        if (tid < REC_B) {
+            const int t = tid;
+            double z = 0.0;
+            if (t < nb) {
+                const double Rt = sh.R[t];
+                for (int w = 0; w < nwave; ++w) {
+                    const double sw = sh.psum[w * REC_B + t];
+                    if (sw > 0.0) z = fma(sw, exp_minimax(Rt - sh.part[w * REC_B + t]), z);
                 }
-                for (; q < qhi; ++q) s += exp(-(rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q] - Rn));
             }
-            sh.part[wave * REC_B + lane] = s;
+            sh.part[t] = z;
         }
         // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)), j < t
         for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
             const int j = idx / REC_B, t = idx % REC_B;
-            if (t < nb && j < t) sh.a[idx] = exp(-(sh.a[idx] + sh.R[j] - sh.R[t]));
+            if (t < nb && j < t) sh.a[idx] = exp_minimax(-(sh.a[idx] + sh.R[j] - sh.R[t]));
         }
         __syncthreads();
         // (2c) in-block forward substitution on wave 0, then U = R - ln z
         if (wave == 0) {
             const int t = lane;
-            double z = 0.0;
-            for (int w = 0; w < nwave; ++w) z += sh.part[w * REC_B + t];
+            double z = sh.part[t];
             double anext = sh.a[t];
             for (int j = 0; j + 1 < nb; ++j) {
                 const double aj = anext;
