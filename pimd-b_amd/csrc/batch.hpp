@@ -564,14 +564,14 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
         if ((wave & 1) == 0) exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s0, gtid, gnthr);
         else exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s1, gtid, gnthr);
         __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), rd.tmp);
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D));
     } else {
         const RecShared sh = rec_carve(lds, N, D);
         const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
         exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
         exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
         __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_shared_doubles(N, D));
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D));
     }
 }
 

@@ -70,6 +70,25 @@ __device__ __forceinline__ double norm2(const double (&d)[D]) {
     return s;
 }
 
+// Lane l receives what lane (l+1) mod 64 held (DPP wave_rol:1).  Call with
+// the whole wave active.
+__device__ __forceinline__ int lane_rotate(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ double lane_rotate(double v) {
+    const int lo = lane_rotate(__double2loint(v));
+    const int hi = lane_rotate(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Squared norm with fused multiply-adds (the pair loop and the exchange's
+// cycle energies on the step path).
+template <int D>
+__device__ __forceinline__ double norm2_fma(const double (&d)[D]) {
+    double s = d[0] * d[0];
+#pragma unroll
+    for (int a = 1; a < D; ++a) s = fma(d[a], d[a], s);
+    return s;
+}
+
 // a * b + c with c in a scalar register.  The compiler turns p = fma(p, r, c)
 // with a loop-invariant c into a copy of c plus v_fmac; a three-operand
 // v_fma_f64 with the constant in an SGPR needs neither.

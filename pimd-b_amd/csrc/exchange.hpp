@@ -8,7 +8,7 @@
 // Storage: bead-major SoA positions, x[(b*D + a)*N + i].  The cycle energy of
 // the cycle through particles u..v (u <= v) is ecyc(u,v) = E^(v+1, v-u+1) of
 // the reference.  On the step path it is recomputed wherever it is needed
-// from beads 1 and P and the double-double prefix sums of the ring's links
+// from beads 1 and P and the (split) prefix sums of the ring's links
 // (Ring, below: O(N) data, staged in LDS).  Only when a caller asks for the
 // energies themselves (or for the observables) is the triangle materialised,
 // at E[v(v+1)/2 + u]: exactly the reference's packed E_kn triangle (E(m,k) at
@@ -118,46 +118,56 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
     const double hi = s.hi + lo;
     return DD{hi, lo - (hi - s.hi)};
 }
-// (a - b) rounded to double
-__device__ __forceinline__ double dd_diff(double ahi, double alo, double bhi, double blo) {
-#pragma clang fp contract(off)
-    const DD s = dd_two_sum(ahi, -bhi);
-    return s.hi + (s.lo + (alo - blo));
-}
 
 // ---- cycle energies from link prefix sums ----
 // With link_j = |x1[j+1] - xP[j]|^2 (min-imaged) and S_v = sum_{j<v} link_j,
 //     ecyc(u,v) = k/2 (S_v - S_u + |x1[u] - xP[v]|^2),
 // the closed form of the reference's recurrence (quadratic_bosonic_exchange.cpp
-// :45-60).  S is kept as a double-double so S_v - S_u keeps full precision
-// however long the ring: any ecyc is a few flops from O(N) data and the
+// :45-60).  S_v is summed as a double-double and stored split as
+// S_v = H_v + L_v, where H_v is S_v rounded to a multiple of the power of two
+// q = 2^(e - 40) (2^e <= S_{N-1} < 2^(e+1), the largest sum) and L_v is the
+// rest, |L_v| <= q/2.  H_v - H_u is then exact (a multiple of q below 2^42 q)
+// and L_v - L_u rounds far below the last bit of the result, so
+// (H_v - H_u) + (L_v - L_u) is S_v - S_u rounded once, in three additions:
+// any ecyc is a few flops from O(N) data however long the ring, and the
 // N(N+1)/2 triangle is never stored on the step path.
 struct Ring {
     const double* x1;    // [D][N] bead 1
     const double* xP;    // [D][N] bead P
-    const double* Shi;   // [N] prefix sums (high and low parts)
-    const double* Slo;
-    double halfk, L, invL;
-    int pbc;
+    const double* Shi;   // [N] H_v, prefix sums rounded to multiples of q
+    const double* Slo;   // [N] L_v = S_v - H_v
+    double halfk, L, invL;   // L = invL = 0 without periodic boundaries
 };
+
+__host__ __device__ inline Ring make_ring(const double* x1, const double* xP, const double* Sh, const double* Sl,
+                                          double k, bool pbc, double L) {
+    return Ring{x1, xP, Sh, Sl, 0.5 * k, pbc ? L : 0.0, pbc ? 1.0 / L : 0.0};
+}
 
 // Minimum image for a squared distance (the image choice at |dx| = L/2 does
 // not change the square); one rounding less than the reference's
-// dx - L floor(dx/L + 1/2) (bosonic_exchange_base.cpp:30-42).
+// dx - L floor(dx/L + 1/2) (bosonic_exchange_base.cpp:30-42).  Without
+// periodic boundaries L = invL = 0 and the result is dx exactly.  Odd in dx
+// (rint rounds half to even), so the image of -dx is minus the image of dx.
 __device__ __forceinline__ double ring_image(double dx, const Ring& r) {
-    return r.pbc ? fma(-r.L, rint(dx * r.invL), dx) : dx;
+    return fma(-r.L, rint(dx * r.invL), dx);
 }
 
-// ecyc(u,v) from the min-imaged displacement x1[u] - xP[v] (either sign).
+// S_v - S_u from the split sums (see above).
+__device__ __forceinline__ double ring_sdiff(double shv, double slv, double shu, double slu) {
+#pragma clang fp contract(off)
+    return (shv - shu) + (slv - slu);
+}
+
 template <int D>
 __device__ __forceinline__ double ring_ecyc(const Ring& r, const double (&d)[D], double shu, double slu, double shv,
                                             double slv) {
-    return r.halfk * (dd_diff(shv, slv, shu, slu) + norm2<D>(d));
+    return r.halfk * (ring_sdiff(shv, slv, shu, slu) + norm2_fma<D>(d));
 }
 
 // S_0..S_{N-1} of the ring by one wave (chunks of ceil(N/64) per lane, then a
-// scan of the chunk totals across lanes); ring.x1 / ring.xP readable, Shi /
-// Slo written (LDS or global).
+// scan of the chunk totals across lanes), stored split as H + L; ring.x1 /
+// ring.xP readable, Shi / Slo written (LDS or global).
 template <int D>
 __device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, double* Shi, double* Slo) {
     const int c = (N + WAVE - 1) / WAVE;
@@ -172,7 +182,7 @@ __device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, doub
                 double d[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) d[a] = ring_image(r.x1[a * N + v + 1] - r.xP[a * N + v], r);
-                acc = dd_add(acc, DD{norm2<D>(d), 0.0});
+                acc = dd_add(acc, DD{norm2_fma<D>(d), 0.0});
             }
         }
     }
@@ -184,12 +194,17 @@ __device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, doub
     }
     DD ex{__shfl_up(inc.hi, 1, WAVE), __shfl_up(inc.lo, 1, WAVE)};
     if (lane == 0) ex = DD{0.0, 0.0};
+    // q from the total (lane 63's inclusive sum, the largest S)
+    const double total = __shfl(inc.hi, WAVE - 1, WAVE);
+    const double q = total > 0.0 ? ldexp(1.0, ilogb(total) - 40) : 1.0;
+    const double invq = 1.0 / q;
     for (int t = 0; t < c; ++t) {
         const int v = v0 + t;
         if (v < N) {
             const DD sv = dd_add(ex, DD{Shi[v], Slo[v]});
-            Shi[v] = sv.hi;
-            Slo[v] = sv.lo;
+            const double h = rint(sv.hi * invq) * q;
+            Shi[v] = h;
+            Slo[v] = (sv.hi - h) + sv.lo;
         }
     }
 }
@@ -208,63 +223,49 @@ struct RecShared {
     double* Slo;       // [N]
 };
 
-__host__ __device__ __forceinline__ size_t rec_shared_doubles(int N, int D) {
-    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B +
-           (size_t)(2 * D + 2) * N;
+// Doubles of the staged ring (beads 1 and P, split prefix sums) and of one
+// direction's recursion workspace.
+__host__ __device__ inline size_t rec_ring_doubles(int N, int D) { return (size_t)(2 * D + 2) * N; }
+__host__ __device__ inline size_t rec_dir_doubles(int N) {
+    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
 }
+// One workgroup running one recursion at a time: the ring, then one workspace.
+__host__ __device__ inline size_t rec_shared_doubles(int N, int D) { return rec_ring_doubles(N, D) + rec_dir_doubles(N); }
 
-__device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
-    RecShared s;
-    double* p = base;
+__device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N) {
     s.U = p; p += N + 1;
     s.lg = p; p += N + 1;
     s.R = p; p += REC_B;
     s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
     s.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
     s.a = p; p += (size_t)REC_B * REC_B;
+}
+
+// Ring first, so that after the recursions everything from
+// base + rec_ring_doubles(N, D) on is free (the exterior forces' scratch).
+__device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
+    RecShared s;
+    double* p = base;
     s.x1 = p; p += (size_t)D * N;
     s.xP = p; p += (size_t)D * N;
     s.Shi = p; p += N;
-    s.Slo = p;
+    s.Slo = p; p += N;
+    rec_carve_dir(s, p, N);
     return s;
 }
 
 // Workspace of the two recursions run side by side by two halves of one
 // workgroup (the batched form): the staged ring, then each direction's own
-// U, ln i, R, partials and in-block matrix.  After the recursions the
-// region from `tmp` on is free (the exterior forces' scratch).
+// U, ln i, R, partials and in-block matrix.
 struct RecDual {
     RecShared s0, s1;
-    double* tmp;
 };
-__host__ __device__ inline size_t rec_dir_doubles(int N) {
-    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
-}
-__host__ __device__ inline size_t rec_dual_doubles(int N, int D) {
-    return (size_t)(2 * D + 2) * N + 2 * rec_dir_doubles(N);
-}
 __device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D) {
     RecDual rd;
-    double* p = base;
-    double* x1 = p; p += (size_t)D * N;
-    double* xP = p; p += (size_t)D * N;
-    double* Shi = p; p += N;
-    double* Slo = p; p += N;
-    rd.tmp = p;
-    RecShared* ss[2] = {&rd.s0, &rd.s1};
-    for (int q = 0; q < 2; ++q) {
-        RecShared& sq = *ss[q];
-        sq.U = p; p += N + 1;
-        sq.lg = p; p += N + 1;
-        sq.R = p; p += REC_B;
-        sq.part = p; p += (size_t)REC_MAXWAVES * REC_B;
-        sq.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
-        sq.a = p; p += (size_t)REC_B * REC_B;
-        sq.x1 = x1;
-        sq.xP = xP;
-        sq.Shi = Shi;
-        sq.Slo = Slo;
-    }
+    rd.s0 = rec_carve(base, N, D);
+    double* p = base + rec_shared_doubles(N, D);
+    rd.s1 = rd.s0;
+    rec_carve_dir(rd.s1, p, N);
     return rd;
 }
 
@@ -278,7 +279,7 @@ __device__ __forceinline__ Ring ring_stage(const RecShared& sh, int N, const dou
         sh.xP[e] = xP[e];
     }
     __syncthreads();
-    Ring r{sh.x1, sh.xP, sh.Shi, sh.Slo, 0.5 * k, L, 1.0 / L, pbc ? 1 : 0};
+    const Ring r = make_ring(sh.x1, sh.xP, sh.Shi, sh.Slo, k, pbc, L);
     if (threadIdx.x < WAVE) {
         ring_prefix<D>(r, N, threadIdx.x, sh.Shi, sh.Slo);
         if (Shi_out) {
@@ -575,20 +576,49 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
 }
 
 // The exterior spring forces of beads 1 and P for one ring on a whole
-// workgroup (the batched form), the same sums as exterior_force arranged
-// lane-per-particle.  Thread t takes particle l = t mod N and the t/N-th of
-// `splits` equal ranges of j.  Every j gives particle l exactly one term of
-// the main loop: the bead-1 term when j >= l-1 (P[j][l], the link at
-// j = l-1), the bead-P term when j < l-1 (P[l][j]); the bead-P terms at
-// j = l-1, l and the bead-P link at j = l+1 are added by the first range.
-// Lanes of a wave share j, so the ring, V, Vb and 1/(i+1) are LDS broadcast
-// reads, and each lane's load is the same length.  Partial sums go through
-// LDS and are added in range order; f[bead][a][l] += k (S + interior link).
+// workgroup (the batched form): the sums of exterior_force with every
+// connection probability evaluated once.  P[b][a] with a <= b (the cycle
+// through particles a..b, closed by the last bead of b and the first bead of
+// a) enters both
+//     F_1[a] += k P[b][a] d   (:188-215)   and   F_P[b] -= k P[b][a] d   (:159-186)
+// with d = xP[b] - x1[a] (min image; the bead-P term's displacement is -d).
+// The pairs a <= b are cut into 64 x 64 tiles (A, B), A <= B, and each tile's
+// 64 lane offsets into chunks of `ch` offsets.  A wave takes one chunk: lane t
+// holds particle a = 64A + t of bead 1 in registers and meets
+// b = 64B + (t + s) mod 64 at offset s, whose data it reads from LDS; its F_1
+// partial stays in registers and the F_P partial of its column moves one lane
+// along the wave after every offset (DPP), so every lane adds to its own
+// column.  In a diagonal tile only b >= a counts.  Each round of one chunk per
+// wave leaves its partials in LDS, where they are added per particle in chunk
+// order, so the result depends on the data only.  The links b = a - 1
+// (probability 1 - exp(...)) and the interior links are added per particle at
+// the end: f[bead][a][l] += k (S + links).
 // Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x).
-__host__ __device__ inline int ext_splits(int N, int nthreads) { return N < nthreads ? nthreads / N : 1; }
 __host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
-    const int sp = ext_splits(N, nthreads);
-    return 3 * (size_t)N + 2 + (sp > 1 ? (size_t)sp * 2 * D * N : 0);
+    const int nw = nthreads / WAVE;
+    return 3 * (size_t)N + 1 + 2 * (size_t)D * N + 2 * (size_t)nw + (size_t)nw * 2 * D * WAVE;
+}
+
+// Chunk length: the longest of 64, 32, 16, 8 offsets whose chunk count fills
+// every round or gives at least four rounds.
+__device__ __forceinline__ int ext_chunk(int ntile, int nw) {
+    int ch = WAVE;
+    while (ch > 8) {
+        const int nc = ntile * (WAVE / ch);
+        if (nc % nw == 0 || nc >= 4 * nw) break;
+        ch >>= 1;
+    }
+    return ch;
+}
+
+// Tile (A, B), A <= B, of index `tile` in row-major order over T x T.
+__device__ __forceinline__ void ext_tile(int tile, int T, int& A, int& B) {
+    A = 0;
+    while (tile >= T - A) {
+        tile -= T - A;
+        ++A;
+    }
+    B = A + tile;
 }
 
 template <int D>
@@ -597,112 +627,116 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                                                    const double* __restrict__ x2, const double* __restrict__ xPm1,
                                                    double* __restrict__ f1, double* __restrict__ fP, double* scratch) {
     const int nthr = blockDim.x, tid = threadIdx.x;
-    double* Vl = scratch;             // [N+1]
-    double* Vbl = Vl + N + 1;         // [N+1]
-    double* inv1 = Vbl + N + 1;       // [N]   1/(i+1)
-    double* part = inv1 + N;          // [splits][2][D][N]
+    const int lane = tid % WAVE, wave = tid / WAVE, nw = nthr / WAVE;
+    double* Vl = scratch;                      // [N+1] V
+    double* Vbn = Vl + N + 1;                  // [N]   Vb[b+1] - V[N]
+    double* inv1 = Vbn + N;                    // [N]   1/(b+1)
+    double* acc = inv1 + N;                    // [2][D][N] sums over the pairs: F_1, F_P (without k)
+    double* tab = acc + 2 * (size_t)D * N;     // [nw][2] this round's tile (A, B) per wave
+    double* part = tab + 2 * (size_t)nw;       // [nw][2][D][64] this round's chunk partials
+    const double VN = V[N];
     for (int i = tid; i <= N; i += nthr) {
         Vl[i] = V[i];
-        Vbl[i] = Vb[i];
-        if (i < N) inv1[i] = 1.0 / (i + 1);
-    }
-    __syncthreads();
-    const int splits = ext_splits(N, nthr);
-    const int jlen = (N + splits - 1) / splits;
-    const double VN = Vl[N];
-    const int lanes_used = splits > 1 ? splits * N : nthr;
-    for (int base = 0; base < N; base += (splits > 1 ? N : nthr)) {
-        const int l = splits > 1 ? tid % N : base + tid;
-        const int sp = splits > 1 ? tid / N : 0;
-        const bool act = tid < lanes_used && l < N && sp < splits;
-        double S0[D], S1[D], x1l[D], xPl[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            S0[a] = 0.0;
-            S1[a] = 0.0;
-            x1l[a] = act ? r.x1[a * N + l] : 0.0;
-            xPl[a] = act ? r.xP[a * N + l] : 0.0;
+        if (i < N) {
+            Vbn[i] = Vb[i + 1] - VN;
+            inv1[i] = 1.0 / (i + 1);
         }
-        if (act) {
-            const double shl = r.Shi[l], sll = r.Slo[l];
-            const double Vl_l = Vl[l], Vb_l1 = Vbl[l + 1], w_l = inv1[l];
-            // bead-1 link (j = l-1): 1 - exp(-bt (V[l] + Vb[l] - V[N]))
-            const double prlink = l >= 1 ? 1.0 - exp_minimax(-bt * (Vl_l + Vbl[l] - VN)) : 0.0;
-            const int j0 = sp * jlen, j1 = min(N, j0 + jlen);
-            for (int j = j0; j < j1; ++j) {
-                const bool selP = j < l - 1;   // bead-P term P[l][j]; else bead-1 term P[j][l]
+    }
+    for (int e = tid; e < 2 * D * N; e += nthr) acc[e] = 0.0;
+    __syncthreads();
+    const int T = (N + WAVE - 1) / WAVE;
+    const int ntile = T * (T + 1) / 2;
+    const int ch = ext_chunk(ntile, nw);
+    const int per_tile = WAVE / ch;
+    const int nchunk = ntile * per_tile;
+    const double nbt = -bt;
+    for (int c0 = 0; c0 < nchunk; c0 += nw) {
+        const int c = c0 + wave;   // wave-uniform
+        if (c < nchunk) {
+            int A, B;
+            ext_tile(c / per_tile, T, A, B);
+            const int s0 = (c % per_tile) * ch;
+            const int a = A * WAVE + lane;
+            const bool aok = a < N;
+            const int ac = aok ? a : N - 1;
+            double xa[D], F1[D], FPr[D];
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) {
+                xa[ax] = r.x1[ax * N + ac];
+                F1[ax] = 0.0;
+                FPr[ax] = 0.0;
+            }
+            const double sha = r.Shi[ac], sla = r.Slo[ac], va = Vl[ac];
+            for (int s = s0; s < s0 + ch; ++s) {
+                const int bl = (lane + s) & (WAVE - 1);
+                const int b = B * WAVE + bl;
+                const bool ok = aok && b < N && (A != B || bl >= lane);
+                const int bc = b < N ? b : N - 1;
                 double d[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    const double o = selP ? r.x1[a * N + j] : r.xP[a * N + j];
-                    const double sf = selP ? xPl[a] : x1l[a];
-                    d[a] = ring_image(o - sf, r);
-                }
-                double pr;
-                if (j == l - 1) {
-                    pr = prlink;
-                } else {
-                    const double shj = r.Shi[j], slj = r.Slo[j];
-                    // ecyc(u, v): bead 1 (u, v) = (l, j), bead P (u, v) = (j, l)
-                    const double e = selP ? ring_ecyc<D>(r, d, shj, slj, shl, sll)
-                                          : ring_ecyc<D>(r, d, shl, sll, shj, slj);
-                    const double A = selP ? Vl[j] : Vl_l;
-                    const double B = selP ? Vb_l1 : Vbl[j + 1];
-                    const double w = selP ? w_l : inv1[j];
-                    pr = w * exp_minimax(-bt * (A + e + B - VN));
-                }
-                const double p0 = selP ? 0.0 : pr, p1 = selP ? pr : 0.0;
+                for (int ax = 0; ax < D; ++ax) d[ax] = ring_image(r.xP[ax * N + bc] - xa[ax], r);
+                const double e = ring_ecyc<D>(r, d, sha, sla, r.Shi[bc], r.Slo[bc]);   // ecyc(a, b)
+                const double ex = exp_minimax(nbt * (e + (va + Vbn[bc])));
+                const double pr = ok ? inv1[bc] * ex : 0.0;
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    S0[a] = fma(p0, d[a], S0[a]);
-                    S1[a] = fma(p1, d[a], S1[a]);
+                for (int ax = 0; ax < D; ++ax) {
+                    F1[ax] = fma(pr, d[ax], F1[ax]);
+                    FPr[ax] = lane_rotate(fma(-pr, d[ax], FPr[ax]));
                 }
             }
-            if (sp == 0) {
-                // bead-P terms j = l-1, l (cycles through j..l) and the link j = l+1
-                for (int j = max(0, l - 1); j <= min(l + 1, N - 1); ++j) {
-                    double d[D];
+            // lane t now holds the F_P partial of column (t + s0 + ch) mod 64
+            double* pw = part + (size_t)wave * 2 * D * WAVE;
+            const int col = (lane + s0 + ch) & (WAVE - 1);
 #pragma unroll
-                    for (int a = 0; a < D; ++a) d[a] = ring_image(r.x1[a * N + j] - xPl[a], r);
-                    double pr;
-                    if (j == l + 1) {
-                        pr = 1.0 - exp_minimax(-bt * (Vl[l + 1] + Vb_l1 - VN));
-                    } else {
-                        const double e = ring_ecyc<D>(r, d, r.Shi[j], r.Slo[j], shl, sll);
-                        pr = w_l * exp_minimax(-bt * (Vl[j] + e + Vb_l1 - VN));
-                    }
-#pragma unroll
-                    for (int a = 0; a < D; ++a) S1[a] = fma(pr, d[a], S1[a]);
-                }
+            for (int ax = 0; ax < D; ++ax) {
+                pw[ax * WAVE + lane] = F1[ax];
+                pw[(D + ax) * WAVE + col] = FPr[ax];
+            }
+            if (lane == 0) {
+                tab[2 * wave] = A;
+                tab[2 * wave + 1] = B;
             }
         }
-        if (splits > 1) {
-            if (act) {
-#pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    part[(((size_t)sp * 2 + 0) * D + a) * N + l] = S0[a];
-                    part[(((size_t)sp * 2 + 1) * D + a) * N + l] = S1[a];
-                }
-            }
-            __syncthreads();
-            if (act && sp == 0) {
-                for (int q = 1; q < splits; ++q) {
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        S0[a] += part[(((size_t)q * 2 + 0) * D + a) * N + l];
-                        S1[a] += part[(((size_t)q * 2 + 1) * D + a) * N + l];
-                    }
-                }
-            }
+        __syncthreads();
+        // this round's partials, added per particle in chunk order
+        const int nc = min(nw, nchunk - c0);
+        for (int e = tid; e < 2 * D * N; e += nthr) {
+            const int which = e / (D * N), rem = e % (D * N);
+            const int ax = rem / N, pt = (rem % N) / WAVE, pl = (rem % N) % WAVE;
+            double sum = acc[e];
+            for (int w = 0; w < nc; ++w)
+                if ((int)tab[2 * w + which] == pt) sum += part[((size_t)w * 2 + which) * D * WAVE + ax * WAVE + pl];
+            acc[e] = sum;
         }
-        if (act && sp == 0) {
+        __syncthreads();
+    }
+    // the links b = a - 1 and the interior links, one particle per thread
+    for (int l = tid; l < N; l += nthr) {
+        double x1l[D], xPl[D], d1[D], dP[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const double s0 = S0[a] + ring_image(x2[a * N + l] - x1l[a], r);
-                const double s1 = S1[a] + ring_image(xPm1[a * N + l] - xPl[a], r);
-                f1[a * N + l] = s0 * k + f1[a * N + l];
-                fP[a * N + l] = s1 * k + fP[a * N + l];
-            }
+        for (int ax = 0; ax < D; ++ax) {
+            x1l[ax] = r.x1[ax * N + l];
+            xPl[ax] = r.xP[ax * N + l];
+            d1[ax] = 0.0;
+            dP[ax] = 0.0;
+        }
+        double p1 = 0.0, pP = 0.0;
+        if (l >= 1) {   // bead 1 of l from bead P of l-1: P[l-1][l] = 1 - exp(-bt (V[l] + Vb[l] - V[N]))
+            p1 = 1.0 - exp_minimax(nbt * (Vl[l] + Vbn[l - 1]));
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) d1[ax] = ring_image(r.xP[ax * N + l - 1] - x1l[ax], r);
+        }
+        if (l + 1 < N) {   // bead P of l to bead 1 of l+1: P[l][l+1] = 1 - exp(-bt (V[l+1] + Vb[l+1] - V[N]))
+            pP = 1.0 - exp_minimax(nbt * (Vl[l + 1] + Vbn[l]));
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) dP[ax] = ring_image(r.x1[ax * N + l + 1] - xPl[ax], r);
+        }
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+            const double s1 = fma(p1, d1[ax], acc[ax * N + l]) + ring_image(x2[ax * N + l] - x1l[ax], r);
+            const double sP = fma(pP, dP[ax], acc[(D + ax) * N + l]) + ring_image(xPm1[ax * N + l] - xPl[ax], r);
+            f1[ax * N + l] = s1 * k + f1[ax * N + l];
+            fP[ax * N + l] = sP * k + fP[ax * N + l];
         }
     }
     __syncthreads();
@@ -725,7 +759,7 @@ static __global__ void __launch_bounds__(256) k_exchange_forces(int N, int P, do
     const size_t bs = (size_t)D * N;
     const double* x2 = x + bs * (1 % P);
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
-    const Ring r{x, x + bs * (P - 1), Shi, Slo, 0.5 * k, L, 1.0 / L, pbc};
+    const Ring r = make_ring(x, x + bs * (P - 1), Shi, Slo, k, pbc != 0, L);
     double F[D];
     exterior_force<D>(which, l, N, lane, bt, k, r, V, Vb, x2, xPm1, false, F);
     if (lane == 0) {
@@ -752,7 +786,7 @@ static __global__ void __launch_bounds__(256) k_exchange_vb_grad(int N, double b
     if (task >= 2 * N) return;
     const int which = task / N, l = task % N;
     const int lane = threadIdx.x % WAVE;
-    const Ring r{x1, xP, Shi, Slo, 0.5 * k, L, 1.0 / L, pbc};
+    const Ring r = make_ring(x1, xP, Shi, Slo, k, pbc != 0, L);
     double F[D];
     exterior_force<D>(which, l, N, lane, bt, k, r, V, Vb, x1, xP, true, F);
     if (lane == 0) {

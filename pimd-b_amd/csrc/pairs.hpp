@@ -71,16 +71,6 @@ __device__ __forceinline__ double rsqrt_refined(double r2) {
     return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
-// Squared norm with fused multiply-adds (the pair loop; the exchange keeps
-// the reference's unfused sum, norm2).
-template <int D>
-__device__ __forceinline__ double norm2_fma(const double (&d)[D]) {
-    double s = d[0] * d[0];
-#pragma unroll
-    for (int a = 1; a < D; ++a) s = fma(d[a], d[a], s);
-    return s;
-}
-
 // The reference's strict test |d| < cutoff (simulation.cpp:444) for the rare
 // r^2 within 1e-12 of cutoff^2, where r^2 alone cannot decide; |d| is sqrt of
 // the same sum of squares the reference takes the norm of.  Out of line so
@@ -107,15 +97,6 @@ __device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&
 // LDS pointer type: accesses through it are ds_ instructions that the
 // compiler knows cannot alias global memory.
 typedef __attribute__((address_space(3))) double lds_double;
-
-// Lane l receives what lane (l+1) mod 64 held (DPP wave_rol:1).  Call with
-// the whole wave active.
-__device__ __forceinline__ int lane_rotate(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xF, 0xF, false); }
-__device__ __forceinline__ double lane_rotate(double v) {
-    const int lo = lane_rotate(__double2loint(v));
-    const int hi = lane_rotate(__double2hiint(v));
-    return __hiloint2double(hi, lo);
-}
 
 // All pairs of one slice on one wave.  xs: slice positions [D][N]; fo: the
 // slice's force array [D][N], owned by this wave, overwritten with the pair

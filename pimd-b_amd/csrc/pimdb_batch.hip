@@ -227,11 +227,12 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         const pimdb_params& p = reps[r];
         if (b->desc[r].bosonic) {
             // both recursions side by side when their workspace fits the LDS
-            const size_t dual = rec_dual_doubles(p.natoms, p.ndim);
+            // (the exterior forces' scratch overlays the recursion workspace)
+            const size_t ring = rec_ring_doubles(p.natoms, p.ndim);
+            const size_t ext = ext_scratch_doubles(p.natoms, p.ndim, 1024);
+            const size_t dual = ring + std::max(2 * rec_dir_doubles(p.natoms), ext);
             b->desc[r].xdual = sizeof(double) * (16 + dual) <= device_lds_limit() ? 1 : 0;
-            need = std::max(need, b->desc[r].xdual ? dual
-                                                   : rec_shared_doubles(p.natoms, p.ndim) +
-                                                         ext_scratch_doubles(p.natoms, p.ndim, 1024));
+            need = std::max(need, b->desc[r].xdual ? dual : ring + std::max(rec_dir_doubles(p.natoms), ext));
         }
         need = std::max(need, batch_nm_lds_doubles(p.nbeads, 1024 / WAVE));
         // pair forces of every wave's slice in LDS when they fit
