@@ -42,6 +42,10 @@ namespace pimdb {
 constexpr int REC_B = 64;   // targets per block: one per lane of the sequential wave
 constexpr int REC_MAXWAVES = 16;
 constexpr int REC_UNROLL = 8;  // independent source loads in flight per lane
+constexpr int REC_PF = 16;     // in-block entries read ahead by the sequential passes
+// Row stride of the in-block matrix: odd, so that a wave reading down a column
+// (the in-block triangle passes) spreads over the LDS banks.
+constexpr int REC_LD = REC_B + 1;
 
 // Row u of the row-major copy ET starts at u N - u(u-1)/2 and holds v = u..N-1.
 __host__ __device__ __forceinline__ size_t et_row(int u, int N) { return (size_t)u * N - (size_t)u * (u - 1) / 2; }
@@ -216,7 +220,7 @@ struct RecShared {
     double* R;         // [REC_B] shifts of the block's targets
     double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min, then the earlier-source sums
     double* psum;      // [REC_MAXWAVES][REC_B] per-wave partial sums scaled to that wave's min
-    double* a;         // [REC_B*REC_B] in-block weights, then coefficients, a[j*REC_B + t]
+    double* a;         // [REC_B*REC_LD] in-block weights, then coefficients, a[j*REC_LD + t]
     double* x1;        // [D][N] bead 1
     double* xP;        // [D][N] bead P
     double* Shi;       // [N] link prefix sums
@@ -227,7 +231,7 @@ struct RecShared {
 // direction's recursion workspace.
 __host__ __device__ inline size_t rec_ring_doubles(int N, int D) { return (size_t)(2 * D + 2) * N; }
 __host__ __device__ inline size_t rec_dir_doubles(int N) {
-    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_B;
+    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_LD;
 }
 // One workgroup running one recursion at a time: the ring, then one workspace.
 __host__ __device__ inline size_t rec_shared_doubles(int N, int D) { return rec_ring_doubles(N, D) + rec_dir_doubles(N); }
@@ -238,7 +242,7 @@ __device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N) {
     s.R = p; p += REC_B;
     s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
     s.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
-    s.a = p; p += (size_t)REC_B * REC_B;
+    s.a = p; p += (size_t)REC_B * REC_LD;
 }
 
 // Ring first, so that after the recursions everything from
@@ -329,6 +333,44 @@ __device__ __forceinline__ double rec_w(const RecShared& sh, const Ring& r, doub
     return bt * e + (DIR == 0 ? t.lg : sh.lg[N - q]);
 }
 
+// Entry (t, j), j < t, of the strict lower triangle of a block taken by lane
+// `lane` of virtual row vr (0 <= vr < REC_B/2): rows vr+1 and REC_B-1-vr
+// together hold REC_B entries, so REC_B/2 virtual rows cover the triangle
+// once (the last one half full; t = -1 for its idle lanes).
+__device__ __forceinline__ void rec_tri(int vr, int lane, int& t, int& j) {
+    if (lane <= vr) {
+        t = vr + 1;
+        j = lane;
+    } else {
+        t = REC_B - 1 - vr;
+        j = lane - vr - 1;
+        if (vr == REC_B / 2 - 1) t = -1;
+    }
+}
+
+// The sequential in-block pass of one wave, lane t holding target t's value
+// v_t: for j = 0 .. REC_B-2 in turn, every lane takes
+//     v_t = min(v_t, a[j][t] + v_j)   (KIND 0, the min-plus pass) or
+//     v_t = v_t + a[j][t] v_j         (KIND 1, the forward substitution),
+// v_j being what lane j holds at step j (final by then: a[j'][t] is +inf,
+// resp. 0, for j' >= t).  Straight-line code, with a[j][t] read REC_PF steps
+// ahead, so each step costs one cross-lane read and one or two operations.
+// Targets past the end of a short block only meet +inf / 0 entries.
+template <int KIND>
+__device__ __forceinline__ double rec_inblock_seq(const double* a, int t, double v) {
+    double buf[REC_PF];
+#pragma unroll
+    for (int u = 0; u < REC_PF; ++u) buf[u] = a[u * REC_LD + t];
+#pragma unroll
+    for (int j = 0; j < REC_B - 1; ++j) {
+        const double aj = buf[j % REC_PF];
+        if (j + REC_PF < REC_B) buf[j % REC_PF] = a[(j + REC_PF) * REC_LD + t];
+        const double vj = wave_bcast_uniform(v, j);
+        v = KIND == 0 ? fmin(v, aj + vj) : fma(aj, vj, v);
+    }
+    return v;
+}
+
 // Blocks of the recursions for N particles; the backward recursion has N-1
 // targets and may leave the last block empty.
 __host__ __device__ inline int rec_blocks(int N) { return (N + REC_B - 1) / REC_B; }
@@ -369,9 +411,20 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         }
         const int tn = n0 + min(lane, nb - 1);   // the lane's target (clamped)
         const RecTarget<D> tg = rec_target<DIR, D>(sh, N, tn);
-        // (0) in-block weights w(n0+t, n0+j), j < t, staged as a[j][t]
-        for (int j = wave; j < REC_B; j += nthr / WAVE)
-            sh.a[j * REC_B + lane] = (lane < nb && j < lane) ? rec_w<DIR, D>(sh, r, bt, N, tg, n0 + j) : 0.0;
+        // (0) in-block weights w(n0+t, n0+j), j < t < nb, staged as a[j][t]
+        // (the strict lower triangle, two rows per wave pass); +inf elsewhere
+        for (int vr = wave; vr < REC_B / 2; vr += nthr / WAVE) {
+            int t, j;
+            rec_tri(vr, lane, t, j);
+            if (t >= 0 && t < nb) {
+                const RecTarget<D> tt = rec_target<DIR, D>(sh, N, n0 + t);
+                sh.a[j * REC_LD + t] = rec_w<DIR, D>(sh, r, bt, N, tt, n0 + j);
+            }
+        }
+        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
+            const int j = idx / REC_B, t = idx % REC_B;
+            if (j >= t || t >= nb) sh.a[j * REC_LD + t] = INFINITY;
+        }
         // (1) one pass over this wave's share of earlier sources: the lane's
         // running minimum m of y_q = w(n,q) + U_q and s = sum_q exp(-(y_q - m)),
         // rescaled whenever a chunk lowers m
@@ -406,20 +459,13 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             sh.psum[wave * REC_B + lane] = s;
         }
         __syncthreads();
-        // (1b) in-block min-plus on wave 0
+        // (1b) in-block min-plus on wave 0 (a[j][t] = +inf for j >= t)
         if (wave == 0) {
             const int t = lane;
             double rr = INFINITY;
             for (int w = 0; w < nwave; ++w) rr = fmin(rr, sh.part[w * REC_B + t]);
             if (t >= nb) rr = INFINITY;
-            double wnext = sh.a[t];
-            for (int j = 0; j + 1 < nb; ++j) {
-                const double wj = wnext;
-                wnext = sh.a[(j + 1) * REC_B + t];
-                const double Rj = wave_bcast_uniform(rr, j);
-                if (t > j) rr = fmin(rr, wj + Rj);
-            }
-            sh.R[t] = rr;
+            sh.R[t] = rec_inblock_seq<0>(sh.a, t, rr);
         }
         __syncthreads();
         // (2a) the earlier sources' sum relative to R_t, sum_w s_w exp(-(m_w - R_t)),
@@ -436,23 +482,23 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             }
             sh.part[t] = z;
         }
-        // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)), j < t
+        // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)),
+        // j < t < nb; 0 elsewhere
+        for (int vr = wave; vr < REC_B / 2; vr += nthr / WAVE) {
+            int t, j;
+            rec_tri(vr, lane, t, j);
+            if (t >= 0 && t < nb) sh.a[j * REC_LD + t] = exp_minimax(-(sh.a[j * REC_LD + t] + sh.R[j] - sh.R[t]));
+        }
         for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
             const int j = idx / REC_B, t = idx % REC_B;
-            if (t < nb && j < t) sh.a[idx] = exp_minimax(-(sh.a[idx] + sh.R[j] - sh.R[t]));
+            if (j >= t || t >= nb) sh.a[j * REC_LD + t] = 0.0;
         }
         __syncthreads();
-        // (2c) in-block forward substitution on wave 0, then U = R - ln z
+        // (2c) in-block forward substitution on wave 0 (a[j][t] = 0 for
+        // j >= t), then U = R - ln z
         if (wave == 0) {
             const int t = lane;
-            double z = sh.part[t];
-            double anext = sh.a[t];
-            for (int j = 0; j + 1 < nb; ++j) {
-                const double aj = anext;
-                anext = sh.a[(j + 1) * REC_B + t];
-                const double zj = wave_bcast_uniform(z, j);
-                z = fma(aj, zj, z);
-            }
+            const double z = rec_inblock_seq<1>(sh.a, t, sh.part[t]);
             if (t < nb) {
                 const double u = sh.R[t] - log(z);
                 sh.U[n0 + t] = u;
@@ -583,35 +629,38 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
 //     F_1[a] += k P[b][a] d   (:188-215)   and   F_P[b] -= k P[b][a] d   (:159-186)
 // with d = xP[b] - x1[a] (min image; the bead-P term's displacement is -d).
 // The pairs a <= b are cut into 64 x 64 tiles (A, B), A <= B, and each tile's
-// 64 lane offsets into chunks of `ch` offsets.  A wave takes one chunk: lane t
-// holds particle a = 64A + t of bead 1 in registers and meets
+// 64 lane offsets into chunks of `ch` offsets.  A wave takes one chunk at a
+// time: lane t holds particle a = 64A + t of bead 1 in registers and meets
 // b = 64B + (t + s) mod 64 at offset s, whose data it reads from LDS; its F_1
 // partial stays in registers and the F_P partial of its column moves one lane
 // along the wave after every offset (DPP), so every lane adds to its own
-// column.  In a diagonal tile only b >= a counts.  Each round of one chunk per
-// wave leaves its partials in LDS, where they are added per particle in chunk
-// order, so the result depends on the data only.  The links b = a - 1
-// (probability 1 - exp(...)) and the interior links are added per particle at
-// the end: f[bead][a][l] += k (S + links).
+// column.  In a diagonal tile only b >= a counts.  Each chunk leaves its
+// partials in its own LDS slot; after a round of up to EXT_SLOTS chunks they
+// are added per particle in chunk order, so the result depends on the data
+// only.  The links b = a - 1 (probability 1 - exp(...)) and the interior links
+// are added per particle at the end: f[bead][a][l] += k (S + links).
 // Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x).
-__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
-    const int nw = nthreads / WAVE;
-    return 3 * (size_t)N + 1 + 2 * (size_t)D * N + 2 * (size_t)nw + (size_t)nw * 2 * D * WAVE;
-}
+constexpr int EXT_SLOTS = 32;
 
-// Chunk length: the longest of 64, 32, 16, 8 offsets whose chunk count fills
-// every round or gives at least four rounds.
-__device__ __forceinline__ int ext_chunk(int ntile, int nw) {
+// Chunk length: the longest of 64, 32, 16, 8 offsets that gives every wave a
+// chunk.
+__host__ __device__ inline int ext_chunk(int N, int nw) {
+    const int T = (N + WAVE - 1) / WAVE, ntile = T * (T + 1) / 2;
     int ch = WAVE;
-    while (ch > 8) {
-        const int nc = ntile * (WAVE / ch);
-        if (nc % nw == 0 || nc >= 4 * nw) break;
-        ch >>= 1;
-    }
+    while (ch > 8 && ntile * (WAVE / ch) < nw) ch >>= 1;
     return ch;
 }
+__host__ __device__ inline int ext_slots(int N, int nw) {
+    const int T = (N + WAVE - 1) / WAVE, ntile = T * (T + 1) / 2;
+    const int nchunk = ntile * (WAVE / ext_chunk(N, nw));
+    return nchunk < EXT_SLOTS ? nchunk : EXT_SLOTS;
+}
+__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
+    const int nw = nthreads / WAVE;
+    return 3 * (size_t)N + 1 + 2 * (size_t)D * N + (size_t)ext_slots(N, nw) * 2 * D * WAVE;
+}
 
-// Tile (A, B), A <= B, of index `tile` in row-major order over T x T.
+// Tile (A, B), A <= B, of index `tile` in row-major order over T x T, and back.
 __device__ __forceinline__ void ext_tile(int tile, int T, int& A, int& B) {
     A = 0;
     while (tile >= T - A) {
@@ -620,6 +669,7 @@ __device__ __forceinline__ void ext_tile(int tile, int T, int& A, int& B) {
     }
     B = A + tile;
 }
+__device__ __forceinline__ int ext_tile_index(int A, int B, int T) { return A * T - A * (A - 1) / 2 + (B - A); }
 
 template <int D>
 __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, const Ring& r,
@@ -632,8 +682,7 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
     double* Vbn = Vl + N + 1;                  // [N]   Vb[b+1] - V[N]
     double* inv1 = Vbn + N;                    // [N]   1/(b+1)
     double* acc = inv1 + N;                    // [2][D][N] sums over the pairs: F_1, F_P (without k)
-    double* tab = acc + 2 * (size_t)D * N;     // [nw][2] this round's tile (A, B) per wave
-    double* part = tab + 2 * (size_t)nw;       // [nw][2][D][64] this round's chunk partials
+    double* part = acc + 2 * (size_t)D * N;    // [slots][2][D][64] one round's chunk partials
     const double VN = V[N];
     for (int i = tid; i <= N; i += nthr) {
         Vl[i] = V[i];
@@ -645,14 +694,13 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
     for (int e = tid; e < 2 * D * N; e += nthr) acc[e] = 0.0;
     __syncthreads();
     const int T = (N + WAVE - 1) / WAVE;
-    const int ntile = T * (T + 1) / 2;
-    const int ch = ext_chunk(ntile, nw);
-    const int per_tile = WAVE / ch;
-    const int nchunk = ntile * per_tile;
+    const int ch = ext_chunk(N, nw), per_tile = WAVE / ch;
+    const int nchunk = T * (T + 1) / 2 * per_tile;
+    const int slots = ext_slots(N, nw);
     const double nbt = -bt;
-    for (int c0 = 0; c0 < nchunk; c0 += nw) {
-        const int c = c0 + wave;   // wave-uniform
-        if (c < nchunk) {
+    for (int c0 = 0; c0 < nchunk; c0 += slots) {
+        const int cend = min(nchunk, c0 + slots);
+        for (int c = c0 + wave; c < cend; c += nw) {   // wave-uniform
             int A, B;
             ext_tile(c / per_tile, T, A, B);
             const int s0 = (c % per_tile) * ch;
@@ -685,27 +733,29 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                 }
             }
             // lane t now holds the F_P partial of column (t + s0 + ch) mod 64
-            double* pw = part + (size_t)wave * 2 * D * WAVE;
+            double* pw = part + (size_t)(c - c0) * 2 * D * WAVE;
             const int col = (lane + s0 + ch) & (WAVE - 1);
 #pragma unroll
             for (int ax = 0; ax < D; ++ax) {
                 pw[ax * WAVE + lane] = F1[ax];
                 pw[(D + ax) * WAVE + col] = FPr[ax];
             }
-            if (lane == 0) {
-                tab[2 * wave] = A;
-                tab[2 * wave + 1] = B;
-            }
         }
         __syncthreads();
-        // this round's partials, added per particle in chunk order
-        const int nc = min(nw, nchunk - c0);
+        // this round's partials, added per particle in chunk order: the
+        // chunks of tiles (A, B >= A) for F_1 of row tile A, of tiles
+        // (A <= B, B) for F_P of column tile B
         for (int e = tid; e < 2 * D * N; e += nthr) {
             const int which = e / (D * N), rem = e % (D * N);
             const int ax = rem / N, pt = (rem % N) / WAVE, pl = (rem % N) % WAVE;
             double sum = acc[e];
-            for (int w = 0; w < nc; ++w)
-                if ((int)tab[2 * w + which] == pt) sum += part[((size_t)w * 2 + which) * D * WAVE + ax * WAVE + pl];
+            for (int o = which == 0 ? pt : 0; o < (which == 0 ? T : pt + 1); ++o) {
+                const int tile = which == 0 ? ext_tile_index(pt, o, T) : ext_tile_index(o, pt, T);
+                for (int sub = 0; sub < per_tile; ++sub) {
+                    const int c = tile * per_tile + sub;
+                    if (c >= c0 && c < cend) sum += part[((size_t)(c - c0) * 2 + which) * D * WAVE + ax * WAVE + pl];
+                }
+            }
             acc[e] = sum;
         }
         __syncthreads();
