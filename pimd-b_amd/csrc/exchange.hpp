@@ -218,8 +218,8 @@ struct RecShared {
     double* U;         // [N+1]
     double* lg;        // [N+1]  ln(i)
     double* R;         // [REC_B] shifts of the block's targets
-    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min, then the earlier-source sums
-    double* psum;      // [REC_MAXWAVES][REC_B] per-wave partial sums scaled to that wave's min
+    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min of the earlier sources
+    double* psum;      // [REC_MAXWAVES][REC_B] per-wave partial sums scaled to that min, then to R
     double* a;         // [REC_B*REC_LD] in-block weights, then coefficients, a[j*REC_LD + t]
     double* x1;        // [D][N] bead 1
     double* xP;        // [D][N] bead P
@@ -328,9 +328,10 @@ __device__ __forceinline__ double rec_w(const RecShared& sh, const Ring& r, doub
     double d[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) d[a] = ring_image(xs[a * N + idx] - t.x[a], r);
-    const double e = DIR == 0 ? ring_ecyc<D>(r, d, sh.Shi[idx], sh.Slo[idx], t.shi, t.slo)
-                              : ring_ecyc<D>(r, d, t.shi, t.slo, sh.Shi[idx], sh.Slo[idx]);
-    return bt * e + (DIR == 0 ? t.lg : sh.lg[N - q]);
+    // beta~ ecyc = (beta~ k/2) (S_v - S_u + |d|^2)
+    const double sd = DIR == 0 ? ring_sdiff(t.shi, t.slo, sh.Shi[idx], sh.Slo[idx])
+                               : ring_sdiff(sh.Shi[idx], sh.Slo[idx], t.shi, t.slo);
+    return fma(bt * r.halfk, sd + norm2_fma<D>(d), DIR == 0 ? t.lg : sh.lg[N - q]);
 }
 
 // Entry (t, j), j < t, of the strict lower triangle of a block taken by lane
@@ -468,19 +469,14 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             sh.R[t] = rec_inblock_seq<0>(sh.a, t, rr);
         }
         __syncthreads();
-        // (2a) the earlier sources' sum relative to R_t, sum_w s_w exp(-(m_w - R_t)),
-        // added in wave order into part[0][t]
-        if (tid < REC_B) {
-            const int t = tid;
-            double z = 0.0;
-            if (t < nb) {
-                const double Rt = sh.R[t];
-                for (int w = 0; w < nwave; ++w) {
-                    const double sw = sh.psum[w * REC_B + t];
-                    if (sw > 0.0) z = fma(sw, exp_minimax(Rt - sh.part[w * REC_B + t]), z);
-                }
-            }
-            sh.part[t] = z;
+        // (2a) each wave's earlier-source sum relative to R_t,
+        // s_w exp(-(m_w - R_t)), in place of s_w (added in wave order by (2c))
+        if (wave < nwave) {
+            const int t = lane;
+            const double sw = sh.psum[wave * REC_B + t];
+            double c = 0.0;
+            if (t < nb && sw > 0.0) c = sw * exp_minimax(sh.R[t] - sh.part[wave * REC_B + t]);
+            sh.psum[wave * REC_B + t] = c;
         }
         // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)),
         // j < t < nb; 0 elsewhere
@@ -498,7 +494,9 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         // j >= t), then U = R - ln z
         if (wave == 0) {
             const int t = lane;
-            const double z = rec_inblock_seq<1>(sh.a, t, sh.part[t]);
+            double z0 = 0.0;
+            for (int w = 0; w < nwave; ++w) z0 += sh.psum[w * REC_B + t];
+            const double z = rec_inblock_seq<1>(sh.a, t, z0);
             if (t < nb) {
                 const double u = sh.R[t] - log(z);
                 sh.U[n0 + t] = u;
@@ -655,9 +653,13 @@ __host__ __device__ inline int ext_slots(int N, int nw) {
     const int nchunk = ntile * (WAVE / ext_chunk(N, nw));
     return nchunk < EXT_SLOTS ? nchunk : EXT_SLOTS;
 }
+// Column record of particle b: xP[b] (D), H_b, L_b, -beta~ (Vb[b+1] - V[N]),
+// 1/(b+1); an odd stride, so that lanes reading consecutive records spread
+// over the LDS banks.
+constexpr int EXT_REC = 9;
 __host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
     const int nw = nthreads / WAVE;
-    return 3 * (size_t)N + 1 + 2 * (size_t)D * N + (size_t)ext_slots(N, nw) * 2 * D * WAVE;
+    return 2 * (size_t)N + 1 + (size_t)EXT_REC * N + 2 * (size_t)D * N + (size_t)ext_slots(N, nw) * 2 * D * WAVE;
 }
 
 // Tile (A, B), A <= B, of index `tile` in row-major order over T x T, and back.
@@ -678,17 +680,26 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                                                    double* __restrict__ f1, double* __restrict__ fP, double* scratch) {
     const int nthr = blockDim.x, tid = threadIdx.x;
     const int lane = tid % WAVE, wave = tid / WAVE, nw = nthr / WAVE;
+    static_assert(D + 4 <= EXT_REC, "column record");
     double* Vl = scratch;                      // [N+1] V
     double* Vbn = Vl + N + 1;                  // [N]   Vb[b+1] - V[N]
-    double* inv1 = Vbn + N;                    // [N]   1/(b+1)
-    double* acc = inv1 + N;                    // [2][D][N] sums over the pairs: F_1, F_P (without k)
+    double* rec = Vbn + N;                     // [N][EXT_REC] column records
+    double* acc = rec + (size_t)EXT_REC * N;   // [2][D][N] sums over the pairs: F_1, F_P (without k)
     double* part = acc + 2 * (size_t)D * N;    // [slots][2][D][64] one round's chunk partials
     const double VN = V[N];
+    const double nbt = -bt, g = -bt * r.halfk;
     for (int i = tid; i <= N; i += nthr) {
         Vl[i] = V[i];
         if (i < N) {
-            Vbn[i] = Vb[i + 1] - VN;
-            inv1[i] = 1.0 / (i + 1);
+            const double vbn = Vb[i + 1] - VN;
+            Vbn[i] = vbn;
+            double* rp = rec + (size_t)i * EXT_REC;
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) rp[ax] = r.xP[ax * N + i];
+            rp[D] = r.Shi[i];
+            rp[D + 1] = r.Slo[i];
+            rp[D + 2] = nbt * vbn;
+            rp[D + 3] = 1.0 / (i + 1);
         }
     }
     for (int e = tid; e < 2 * D * N; e += nthr) acc[e] = 0.0;
@@ -697,7 +708,6 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
     const int ch = ext_chunk(N, nw), per_tile = WAVE / ch;
     const int nchunk = T * (T + 1) / 2 * per_tile;
     const int slots = ext_slots(N, nw);
-    const double nbt = -bt;
     for (int c0 = 0; c0 < nchunk; c0 += slots) {
         const int cend = min(nchunk, c0 + slots);
         for (int c = c0 + wave; c < cend; c += nw) {   // wave-uniform
@@ -714,25 +724,37 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                 F1[ax] = 0.0;
                 FPr[ax] = 0.0;
             }
-            const double sha = r.Shi[ac], sla = r.Slo[ac], va = Vl[ac];
-            for (int s = s0; s < s0 + ch; ++s) {
+            const double sha = r.Shi[ac], sla = r.Slo[ac], ca = nbt * Vl[ac];
+            // P[b][a] d for this lane's pair at offset s; `masked` tiles (the
+            // diagonal, the ragged last column tile) drop b < a and b >= N
+            auto step = [&](int s, bool masked) {
                 const int bl = (lane + s) & (WAVE - 1);
-                const int b = B * WAVE + bl;
-                const bool ok = aok && b < N && (A != B || bl >= lane);
-                const int bc = b < N ? b : N - 1;
+                int b = B * WAVE + bl;
+                bool ok = true;
+                if (masked) {
+                    ok = aok && b < N && (A != B || bl >= lane);
+                    b = b < N ? b : N - 1;
+                }
+                const double* rp = rec + (size_t)b * EXT_REC;
                 double d[D];
 #pragma unroll
-                for (int ax = 0; ax < D; ++ax) d[ax] = ring_image(r.xP[ax * N + bc] - xa[ax], r);
-                const double e = ring_ecyc<D>(r, d, sha, sla, r.Shi[bc], r.Slo[bc]);   // ecyc(a, b)
-                const double ex = exp_minimax(nbt * (e + (va + Vbn[bc])));
-                const double pr = ok ? inv1[bc] * ex : 0.0;
+                for (int ax = 0; ax < D; ++ax) d[ax] = ring_image(rp[ax] - xa[ax], r);
+                // -beta~ (ecyc(a, b) + V[a] + Vb[b+1] - V[N])
+                const double arg = fma(g, ring_sdiff(rp[D], rp[D + 1], sha, sla) + norm2_fma<D>(d), ca + rp[D + 2]);
+                double pr = rp[D + 3] * exp_minimax(arg);
+                if (masked) pr = ok ? pr : 0.0;
 #pragma unroll
                 for (int ax = 0; ax < D; ++ax) {
                     F1[ax] = fma(pr, d[ax], F1[ax]);
                     FPr[ax] = lane_rotate(fma(-pr, d[ax], FPr[ax]));
                 }
+            };
+            if (A != B && B * WAVE + WAVE <= N) {
+                for (int s = s0; s < s0 + ch; ++s) step(s, false);
+            } else {
+                for (int s = s0; s < s0 + ch; ++s) step(s, true);
             }
-            // lane t now holds the F_P partial of column (t + s0 + ch) mod 64
+        // lane t now holds the F_P partial of column (t + s0 + ch) mod 64
             double* pw = part + (size_t)(c - c0) * 2 * D * WAVE;
             const int col = (lane + s0 + ch) & (WAVE - 1);
 #pragma unroll
