@@ -118,11 +118,11 @@ __host__ __device__ inline int nm_mfma_groups(int P, int nwaves) {
     const int mb = nm_pad16(P) / 16;
     return nwaves >= mb ? nwaves / mb : 1;
 }
-// LDS doubles of the matrix-core path: both matrices padded to Pp x Pp and
-// two column tiles [groups][Pp][16].
+// LDS doubles of the matrix-core path: C^T padded to Pp x Pp and the noise
+// tile [groups][Pp][16].
 __host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
     const size_t Pp = (size_t)nm_pad16(P);
-    return 2 * Pp * Pp + 2 * (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
+    return Pp * Pp + (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
 }
 
 // Element loops over a replica's [bead][axis][particle] arrays take
@@ -177,82 +177,39 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
 #pragma unroll
     for (int a = 0; a < D; ++a) csum[a] = 0.0;
     if (r.thermostat == 1 && r.nm && P <= NM_MFMA_MAXP) {
-        // Matrix-core form.  LDS: Cl[j][k] = C[k][j] and CTl[k][b] = CT[b][k]
-        // (Pp x Pp, zero padded), pc[g][j][c] the staged columns (after the
-        // half kick) and pn[g][k][c] the thermostatted modes, g = column block
-        // of the round.  Wave (mb, g) computes 16 modes of 16 columns, then 16
-        // beads of them.  Its A rows are permuted so that lane q holds the
-        // four consecutive modes 16mb + 4q .. 4q+3: modes 2m and 2m+1 of a
-        // Box-Muller pair are on one lane.
+        // Matrix-core form.  One friction serves every mode (langevin.cpp:15-27,
+        // thermostat_coupling.cpp:27-47), and C is orthogonal, so the O-step
+        //     p' = C^T (c1 C p + c2 xi) = c1 p + C^T (c2 xi)
+        // transforms only the noise.  xi, drawn per mode with the keys of the
+        // single-system form (modes 2m and 2m+1 from one Box-Muller pair), goes
+        // to LDS as pn[g][k][c] (g = column block of the round); wave (bb, g)
+        // multiplies 16 beads of C^T by it on the matrix cores and adds
+        // c1 (p + dt/2 f) in the epilogue.  The same p' as the two transforms
+        // up to rounding.  LDS: CTl[k][b] = CT[b][k] (Pp x Pp, zero padded).
         const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
         const int wave = uni((int)(threadIdx.x / WAVE));
         const int Pp = nm_pad16(P), MB = Pp / 16, KK = Pp / 4;
         const int G = nm_mfma_groups(P, nw), ntask = MB * G;
         const int NCB = (ncols + 15) / 16;
-        lds_double* Cl = (lds_double*)lds;
-        lds_double* CTl = Cl + (size_t)Pp * Pp;
-        lds_double* pc = CTl + (size_t)Pp * Pp;
-        lds_double* pn = pc + (size_t)G * Pp * 16;
-        const double* Ct = uni(bp.Ct) + r.off_C;
+        lds_double* CTl = (lds_double*)lds;
+        lds_double* pn = CTl + (size_t)Pp * Pp;
         const double* CTt = uni(bp.CTt) + r.off_C;
         for (int e = threadIdx.x; e < Pp * Pp; e += blockDim.x) {
             const int a = e / Pp, b = e % Pp;
-            const bool in = a < P && b < P;
-            Cl[e] = in ? Ct[a * P + b] : 0.0;
-            CTl[e] = in ? CTt[a * P + b] : 0.0;
+            CTl[e] = (a < P && b < P) ? CTt[a * P + b] : 0.0;
         }
         const int q = lane / 16, c16 = lane % 16;
-        const int arow = 4 * (c16 % 4) + c16 / 4;   // permuted A row of this lane
-        const int width = G * 16;
+        const int half = (P + 1) / 2, width = G * 16, npair = (Pp / 2) * width;
         for (int cb0 = 0; cb0 < NCB; cb0 += G) {
             const int c0 = cb0 * 16;
-            const int nst = Pp * width, nthr = blockDim.x;
-            for (int base = threadIdx.x; base < nst; base += STREAM_UNROLL * nthr) {
-                double pv[STREAM_UNROLL], fv[STREAM_UNROLL];
-#pragma unroll
-                for (int u = 0; u < STREAM_UNROLL; ++u) {
-                    const int e = base + u * nthr;
-                    const int j = e / width, c = c0 + e % width;
-                    pv[u] = 0.0;
-                    fv[u] = 0.0;
-                    if (e < nst && j < P && c < ncols) {
-                        const size_t g = (size_t)j * ncols + c;
-                        pv[u] = p[g];
-                        if (kick) fv[u] = f[g];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < STREAM_UNROLL; ++u) {
-                    const int e = base + u * nthr;
-                    if (e < nst) {
-                        const int j = e / width, gc = e % width;
-                        pc[((size_t)(gc / 16) * Pp + j) * 16 + gc % 16] = kick ? pv[u] + half_dt * fv[u] : pv[u];
-                    }
-                }
-            }
-            __syncthreads();
-            for (int task = wave; task < ntask; task += nw) {
-                const int mb = task % MB, g = task / MB;
-                if (cb0 + g >= NCB) continue;
-                const lds_double* B = pc + (size_t)g * Pp * 16;
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
-                for (int kk = 0; kk < KK; ++kk) {
-                    const int k = 4 * kk + q;
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Cl[(size_t)k * Pp + mb * 16 + arow],
-                                                                B[k * 16 + c16], acc, 0, 0, 0);
-                }
-                // acc[v] = mode 16mb + 4q + v of column c0 + 16g + c16
-                const int col = c0 + g * 16 + c16;
-                lds_double* out = pn + (size_t)g * Pp * 16;
-#pragma unroll
-                for (int h = 0; h < 4; h += 2) {
-                    const int k = mb * 16 + 4 * q + h;
-                    double g0 = 0.0, g1 = 0.0;
-                    if (k < P && col < ncols)
-                        gauss2((uint32_t)((size_t)(k >> 1) * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
-                    out[k * 16 + c16] = k < P ? r.c1 * acc[h] + r.c2 * g0 : 0.0;
-                    out[(k + 1) * 16 + c16] = k + 1 < P ? r.c1 * acc[h + 1] + r.c2 * g1 : 0.0;
-                }
+            for (int e = threadIdx.x; e < npair; e += blockDim.x) {
+                const int m = e / width, gc = e % width, col = c0 + gc;
+                double g0 = 0.0, g1 = 0.0;
+                if (m < half && col < ncols)
+                    gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                lds_double* out = pn + (size_t)(gc / 16) * Pp * 16 + gc % 16;
+                out[(2 * m) * 16] = 2 * m < P ? r.c2 * g0 : 0.0;
+                out[(2 * m + 1) * 16] = 2 * m + 1 < P ? r.c2 * g1 : 0.0;
             }
             __syncthreads();
             for (int task = wave; task < ntask; task += nw) {
@@ -265,25 +222,37 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CTl[(size_t)k * Pp + bb * 16 + c16],
                                                                 B[k * 16 + c16], acc, 0, 0, 0);
                 }
-                // acc[v] = bead 16bb + q + 4v of column c0 + 16g + c16
+                // acc[v] = (C^T c2 xi) of bead 16bb + q + 4v, column c0 + 16g + c16
                 const int col = c0 + g * 16 + c16;
                 if (col < ncols) {
                     const int ax = col / N;
+                    double pv[4], fv[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int b = bb * 16 + q + 4 * v;
+                        pv[v] = 0.0;
+                        fv[v] = 0.0;
+                        if (b < P) {
+                            pv[v] = p[(size_t)b * ncols + col];
+                            if (kick) fv[v] = f[(size_t)b * ncols + col];
+                        }
+                    }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int b = bb * 16 + q + 4 * v;
                         if (b < P) {
-                            p[(size_t)b * ncols + col] = acc[v];
+                            const double pk = kick ? pv[v] + half_dt * fv[v] : pv[v];
+                            const double pnew = r.c1 * pk + acc[v];
+                            p[(size_t)b * ncols + col] = pnew;
 #pragma unroll
                             for (int a = 0; a < D; ++a)
-                                if (a == ax) csum[a] += acc[v];
+                                if (a == ax) csum[a] += pnew;
                         }
                     }
                 }
             }
-            // the next round's staging waits at its barrier for these reads
+            __syncthreads();   // pn is rewritten by the next round
         }
-        __syncthreads();
     } else if (r.thermostat != 1 || !r.nm) {
         // beads 2m and 2m+1 of a column share one Box-Muller pair
         const int half = (P + 1) / 2, n2 = half * ncols, nthr = blockDim.x;
