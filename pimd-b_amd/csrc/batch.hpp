@@ -24,6 +24,7 @@ struct RepDesc {
     int thermostat, nm, fixcom;
     int xdual;                  // exchange recursions side by side (LDS permitting)
     int plds;                   // pair forces accumulated in LDS (LDS permitting)
+    int fused;                  // normal-mode O-step on the matrix cores with B + A fused in (see wg_momenta)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
     double int_k, strength, cutoff;
@@ -42,6 +43,7 @@ struct BatchPtrs {
     const int* sched_off;
     int* status;               // per replica: 1 when the exchange overflowed
     unsigned long long* ctl;   // [0] step counter, [1] finished-workgroup counter
+    double* cmp;               // per replica [8]: pending COM momentum [0..3], known sum of p [4..7]
 };
 
 // Sum over the workgroup of one double per thread; result in every thread.
@@ -84,7 +86,7 @@ __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
     u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.xdual = uni(r.xdual);
-    u.plds = uni(r.plds);
+    u.plds = uni(r.plds); u.fused = uni(r.fused);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
     u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
@@ -160,11 +162,26 @@ __device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double
 // thermostat_coupling.cpp, simulation.cpp:581-603).  With fixcom the COM
 // momentum per axis is returned in cm and, when `subtract`, removed from p
 // here; otherwise the caller removes it (wg_kick_drift fuses it).
+//
+// Replicas with r.fused (normal-mode Langevin on the matrix cores) go
+// further.  The second O-step of a step leaves its COM momentum pending in
+// cmp[0..D) instead of a pass over p; the first O-step of the next step
+// subtracts it from p on reading.  The first O-step also does B + A in its
+// epilogue (p -= COM, p += dt/2 f, x += dt p / m; wg_kick_drift's
+// arithmetic), with the COM of its result known before the pass: the sum of
+// C^T xi over the beads of a column is sqrt(P) xi_0 (only the centroid row of
+// C has a nonzero sum), so the COM is (c1 S + c2 sqrt(P) sum_i xi_0) / (N P),
+// S the sum of p before the step (cmp[4..]: 0 after a deferred subtraction,
+// the sum of the state a caller set).  p then reads back through
+// pimdb_batch_get_state with the pending COM removed.
 template <int D>
 __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, bool kick_, uint32_t step_,
-                                        uint32_t ostep_, double* red_, double* lds_, bool subtract_, double (&cm)[D]) {
+                                        uint32_t ostep_, double* red_, double* lds_, bool subtract_, double (&cm)[D],
+                                        double* cmp_) {
     const RepDesc r = rep_uniform(r_);
     const bool kick = uni((int)kick_) != 0, subtract = uni((int)subtract_) != 0;
+    double* cmp = uni(cmp_);
+    const bool drift = r.fused && !kick;
     const uint32_t step = (uint32_t)uni((int)step_), ostep = (uint32_t)uni((int)ostep_);
     double* red = uni(red_);
     double* lds = uni(lds_);
@@ -198,6 +215,33 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             const int a = e / Pp, b = e % Pp;
             CTl[e] = (a < P && b < P) ? CTt[a * P + b] : 0.0;
         }
+        // first O-step: the pending COM and the COM of the result
+        double cmin[D], cm1[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            cmin[a] = 0.0;
+            cm1[a] = 0.0;
+        }
+        if (drift && r.fixcom) {
+            double s0[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) s0[a] = 0.0;
+            for (int col = threadIdx.x; col < ncols; col += blockDim.x) {
+                double g0, g1;
+                gauss2((uint32_t)col, step, ostep, r.rid, r.seed, g0, g1);   // mode 0 of column col
+                const int ax = col / N;
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+                    if (a == ax) s0[a] += g0;
+            }
+            const double sq = sqrt((double)P);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                cmin[a] = uni(cmp[a]);
+                cm1[a] = (r.c1 * uni(cmp[4 + a]) + r.c2 * sq * wg_sum(s0[a], red)) / (double)(N * P);
+            }
+        }
+        double* x = uni(bp.x) + r.off_x;
         const int q = lane / 16, c16 = lane % 16;
         const int half = (P + 1) / 2, width = G * 16, npair = (Pp / 2) * width;
         for (int cb0 = 0; cb0 < NCB; cb0 += G) {
@@ -226,27 +270,47 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                 const int col = c0 + g * 16 + c16;
                 if (col < ncols) {
                     const int ax = col / N;
-                    double pv[4], fv[4];
+                    double cmi = 0.0, cmo = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        if (a == ax) {
+                            cmi = cmin[a];
+                            cmo = cm1[a];
+                        }
+                    }
+                    double pv[4], fv[4], xv[4];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int b = bb * 16 + q + 4 * v;
                         pv[v] = 0.0;
                         fv[v] = 0.0;
+                        xv[v] = 0.0;
                         if (b < P) {
-                            pv[v] = p[(size_t)b * ncols + col];
-                            if (kick) fv[v] = f[(size_t)b * ncols + col];
+                            const size_t e = (size_t)b * ncols + col;
+                            pv[v] = p[e];
+                            if (kick || drift) fv[v] = f[e];
+                            if (drift) xv[v] = x[e];
                         }
                     }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int b = bb * 16 + q + 4 * v;
                         if (b < P) {
-                            const double pk = kick ? pv[v] + half_dt * fv[v] : pv[v];
-                            const double pnew = r.c1 * pk + acc[v];
-                            p[(size_t)b * ncols + col] = pnew;
+                            const size_t e = (size_t)b * ncols + col;
+                            if (drift) {
+                                const double pnew = r.c1 * (pv[v] - cmi) + acc[v];
+                                double w = pnew - cmo;
+                                w = w + half_dt * fv[v];
+                                p[e] = w;
+                                x[e] = xv[v] + r.dt * w / r.mass;
+                            } else {
+                                const double pk = kick ? pv[v] + half_dt * fv[v] : pv[v];
+                                const double pnew = r.c1 * pk + acc[v];
+                                p[e] = pnew;
 #pragma unroll
-                            for (int a = 0; a < D; ++a)
-                                if (a == ax) csum[a] += pnew;
+                                for (int a = 0; a < D; ++a)
+                                    if (a == ax) csum[a] += pnew;
+                            }
                         }
                     }
                 }
@@ -375,10 +439,21 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             __syncthreads();
         }
     }
-    if (r.fixcom) {
+    if (r.fixcom && drift) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) cm[a] = 0.0;   // removed in the epilogue
+    } else if (r.fixcom) {
 #pragma unroll
         for (int a = 0; a < D; ++a) cm[a] = wg_sum(csum[a], red) / (double)(N * P);
-        if (subtract) {
+        if (r.fused) {   // left pending for the next step's first O-step
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    cmp[a] = cm[a];
+                    cmp[4 + a] = 0.0;
+                }
+            }
+        } else if (subtract) {
             __syncthreads();
             wg_sub_com<D>(p, N, P, cm);
         }
@@ -563,8 +638,8 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         constexpr bool integ = MODE == BM_STEP || MODE == BM_INTEGRATOR;
         if constexpr (integ) {
             double cm[D];
-            wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm);
-            wg_kick_drift<D>(r, bp, cm);
+            wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
+            if (!r.fused) wg_kick_drift<D>(r, bp, cm);
         }
         if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
@@ -572,7 +647,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
                            bp.Vb + r.off_V, bp.status + ri, lds, r.xdual);
         if constexpr (integ) {
             double cm[D];
-            wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm);
+            wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
         }
     }
     if constexpr (MODE == BM_STEP) {

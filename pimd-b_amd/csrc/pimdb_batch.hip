@@ -212,6 +212,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         RepDesc d{};
         d.N = p.natoms; d.P = p.nbeads; d.bosonic = dv.bosonic; d.pbc = p.pbc;
         d.thermostat = p.thermostat; d.nm = p.nmthermostat; d.fixcom = p.fixcom;
+        d.fused = (p.thermostat == PIMDB_THERMO_LANGEVIN && p.nmthermostat && p.nbeads <= NM_MFMA_MAXP) ? 1 : 0;
         d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
         d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
         d.ext = dv.ext; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
@@ -252,6 +253,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     double *Ct = nullptr, *CTt = nullptr;
     alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
     alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
+    alloc(&b->bp.cmp, 8 * (size_t)nrep);
     alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
     b->bp.Ct = Ct;
     b->bp.CTt = CTt;
@@ -294,7 +296,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->s) (void)hipStreamSynchronize(b->s);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.Vb,
+    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.Vb, b->bp.cmp,
                       const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
@@ -324,6 +326,12 @@ int pimdb_batch_set_state(pimdb_batch* b, int rep, const double* x, const double
         BCK(hipMemcpyAsync(dst[k] + b->desc[rep].off_x, st.data(), sizeof(double) * n, hipMemcpyHostToDevice, b->s));
         BCK(hipStreamSynchronize(b->s));
     }
+    if (p) {
+        // no COM pending; the sum of p per axis for the next first O-step
+        double cmp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (size_t e = 0; e < n; ++e) cmp[4 + e % q.ndim] += p[e];
+        BCK(hipMemcpy(b->bp.cmp + 8 * (size_t)rep, cmp, sizeof(cmp), hipMemcpyHostToDevice));
+    }
     return PIMDB_OK;
 }
 
@@ -339,6 +347,12 @@ int pimdb_batch_get_state(pimdb_batch* b, int rep, double* x, double* p, double*
         BCK(hipMemcpyAsync(st.data(), src[k] + b->desc[rep].off_x, sizeof(double) * n, hipMemcpyDeviceToHost, b->s));
         BCK(hipStreamSynchronize(b->s));
         from_soa(st.data(), dst[k], q.natoms, q.nbeads, q.ndim);
+    }
+    if (p) {
+        // the COM momentum still pending from the last O-step
+        double cmp[8];
+        BCK(hipMemcpy(cmp, b->bp.cmp + 8 * (size_t)rep, sizeof(cmp), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < n; ++e) p[e] -= cmp[e % q.ndim];
     }
     return PIMDB_OK;
 }
@@ -415,6 +429,8 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
     BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
+    std::vector<double> scmp(8 * (size_t)b->R);
+    BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
     int rc = launch_batch(b, modes[phase]);   // warm
     if (!rc) {
         BCK(hipEventRecord(b->t0, b->s));
@@ -430,6 +446,7 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);
     (void)hipFree(sp);
     (void)hipFree(sf);
@@ -452,6 +469,8 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
+    std::vector<double> scmp(8 * (size_t)b->R);
+    BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
     const int cand_threads[3] = {256, 512, 1024};
     double best = 1e300;
     int bt = b->threads, br = b->resident;
@@ -477,6 +496,7 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
+    BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);
     (void)hipFree(sp);
     (void)hipFree(sf);
