@@ -330,6 +330,71 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     return c;
 }
 
+// ln u for u in (0, 1] (Box-Muller's radius).  u = m 2^e with m in
+// [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716,
+// by its Taylor series sum_k s^(2k+1)/(2k+1) through s^21 (the rest is below
+// 1e-18 relative); e ln 2 with ln 2 in the two parts of exp_minimax.
+__device__ __forceinline__ double log_unit(double u) {
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    int e = __builtin_amdgcn_frexp_exp(u);
+    double m = __builtin_amdgcn_frexp_mant(u);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double p = 1.0 / 21.0;
+    p = fma_sconst(p, z, 1.0 / 19.0);
+    p = fma_sconst(p, z, 1.0 / 17.0);
+    p = fma_sconst(p, z, 1.0 / 15.0);
+    p = fma_sconst(p, z, 1.0 / 13.0);
+    p = fma_sconst(p, z, 1.0 / 11.0);
+    p = fma_sconst(p, z, 1.0 / 9.0);
+    p = fma_sconst(p, z, 1.0 / 7.0);
+    p = fma_sconst(p, z, 1.0 / 5.0);
+    p = fma_sconst(p, z, 1.0 / 3.0);
+    p = fma_sconst(p, z, 1.0);
+    const double ed = (double)e;
+    return fma(ed, LN2_HI, fma(ed, LN2_LO, (2.0 * s) * p));
+}
+
+// sin and cos of 2 pi u for u in [0, 1): 4u = q + r with q = rint(4u) and
+// |r| <= 1/2 (both exact), x = r pi/2, and the Taylor series of sin x through
+// x^17 and of cos x through x^18 (|x| <= pi/4: the rest is below 1e-17),
+// rotated by the quadrant q.
+__device__ __forceinline__ void sincos_2pi(double u, double& sn, double& cs) {
+    const double t = 4.0 * u;
+    const double q = rint(t);
+    const double x = (t - q) * 1.57079632679489661923;
+    const double z = x * x;
+    double ps = -1.0 / 355687428096000.0;          // -1/17!
+    ps = fma_sconst(ps, z, 1.0 / 1307674368000.0);        //  1/15!
+    ps = fma_sconst(ps, z, -1.0 / 6227020800.0);          // -1/13!
+    ps = fma_sconst(ps, z, 1.0 / 39916800.0);             //  1/11!
+    ps = fma_sconst(ps, z, -1.0 / 362880.0);              // -1/9!
+    ps = fma_sconst(ps, z, 1.0 / 5040.0);                 //  1/7!
+    ps = fma_sconst(ps, z, -1.0 / 120.0);                 // -1/5!
+    ps = fma_sconst(ps, z, 1.0 / 6.0);                    //  1/3!, sign below
+    const double sx = fma(-x * z, ps, x);
+    double pc = 1.0 / 6402373705728000.0;          //  1/18!
+    pc = fma_sconst(pc, z, -1.0 / 20922789888000.0);      // -1/16!
+    pc = fma_sconst(pc, z, 1.0 / 87178291200.0);          //  1/14!
+    pc = fma_sconst(pc, z, -1.0 / 479001600.0);           // -1/12!
+    pc = fma_sconst(pc, z, 1.0 / 3628800.0);              //  1/10!
+    pc = fma_sconst(pc, z, -1.0 / 40320.0);               // -1/8!
+    pc = fma_sconst(pc, z, 1.0 / 720.0);                  //  1/6!
+    pc = fma_sconst(pc, z, -1.0 / 24.0);                  // -1/4!
+    pc = fma_sconst(pc, z, 0.5);                          //  1/2!, sign below
+    const double cx = fma(-z, pc, 1.0);
+    const int qi = (int)q & 3;
+    const double a = (qi & 1) ? cx : sx, b = (qi & 1) ? sx : cx;
+    sn = (qi & 2) ? -a : a;
+    cs = ((qi + 1) & 2) ? -b : b;
+}
+
 // Two standard normal deviates from one Philox block (Box-Muller on 53-bit
 // uniforms in (0,1]).  Counter = (element, step, ostep|stream, replica).
 __device__ __forceinline__ void gauss2(uint32_t elem, uint32_t step, uint32_t ostep,
@@ -340,9 +405,9 @@ __device__ __forceinline__ void gauss2(uint32_t elem, uint32_t step, uint32_t os
     const uint64_t b = ((uint64_t)r.z << 21) ^ (uint64_t)(r.w >> 11);
     const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) / TWO53;   // (0, 1]
     const double u2 = (double)(b & ((1ull << 53) - 1)) / TWO53;           // [0, 1)
-    const double rad = sqrt(-2.0 * log(u1));
+    const double rad = sqrt(-2.0 * log_unit(u1));
     double s, c;
-    sincospi(2.0 * u2, &s, &c);
+    sincos_2pi(u2, s, c);
     g0 = rad * c;
     g1 = rad * s;
 }

@@ -2,7 +2,8 @@
 // exp throughput (exp_minimax, the exp of the pair kernels), and HBM copy
 // bandwidth.  The copy kernel also serves as the known-size reference kernel
 // when hardware byte counters are scaled.  pimdb_math_probe evaluates the
-// device elementary functions of the pair kernels on caller data.
+// device elementary functions of the pair kernels and of the Gaussian noise
+// on caller data.
 #include <hip/hip_runtime.h>
 
 #include "pairs.hpp"
@@ -44,7 +45,18 @@ __global__ void __launch_bounds__(256) k_probe_math(int fn, const double* __rest
                                                     long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    y[i] = fn == 0 ? pimdb::exp_minimax(x[i]) : pimdb::rsqrt_refined(x[i]);
+    double v;
+    switch (fn) {
+    case 0: v = pimdb::exp_minimax(x[i]); break;
+    case 1: v = pimdb::rsqrt_refined(x[i]); break;
+    case 2: v = pimdb::log_unit(x[i]); break;
+    default: {
+        double sn, cs;
+        pimdb::sincos_2pi(x[i], sn, cs);
+        v = fn == 3 ? sn : cs;
+    }
+    }
+    y[i] = v;
 }
 
 __global__ void __launch_bounds__(256) k_probe_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
@@ -113,7 +125,7 @@ extern "C" int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, doub
 }
 
 extern "C" int pimdb_math_probe(int fn, const double* x, double* y, long long n) {
-    if ((fn != 0 && fn != 1) || n < 0 || (n > 0 && (!x || !y))) return PIMDB_E_ARG;
+    if (fn < 0 || fn > 4 || n < 0 || (n > 0 && (!x || !y))) return PIMDB_E_ARG;
     if (n == 0) return PIMDB_OK;
     double *dx = nullptr, *dy = nullptr;
     int rc = PIMDB_OK;

@@ -303,11 +303,14 @@ def peak_probe(copy_bytes=1 << 30):
 
 
 def math_probe(fn, x):
-    """The pair kernels' device e^x (fn "exp") or 1/sqrt(x) (fn "rsqrt")
-    evaluated on the GPU at the values x."""
+    """The device elementary functions evaluated on the GPU at the values x:
+    the pair kernels' e^x (fn "exp") and 1/sqrt(x) ("rsqrt"), and the noise's
+    ln x on (0, 1] ("log_unit") and sin / cos of 2 pi x on [0, 1)
+    ("sin_2pi", "cos_2pi")."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
-    rc = lib().pimdb_math_probe({"exp": 0, "rsqrt": 1}[fn], _p(x), _p(y), int(x.size))
+    fns = {"exp": 0, "rsqrt": 1, "log_unit": 2, "sin_2pi": 3, "cos_2pi": 4}
+    rc = lib().pimdb_math_probe(fns[fn], _p(x), _p(y), int(x.size))
     if rc != OK:
         _raise(rc, "pimdb_math_probe failed")
     return y

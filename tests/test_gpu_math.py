@@ -53,3 +53,33 @@ def test_rsqrt_within_one_ulp():
     ref = correctly_rounded(lambda v: 1 / mp.sqrt(v), x)
     err = ulp_err(y, ref)
     assert err.max() <= 1.0, (x[np.argmax(err)], err.max())
+
+
+def test_log_unit_within_two_ulp():
+    """ln u on (0, 1], the Box-Muller radius of the device noise."""
+    rng = np.random.default_rng(13)
+    x = np.concatenate([rng.uniform(0.0, 1.0, 1500), 2.0 ** -rng.uniform(0, 53, 1000),
+                        [1.0, np.nextafter(1.0, 0.0), 0.5, np.sqrt(0.5), np.nextafter(np.sqrt(0.5), 1.0),
+                         2.0 ** -53, 1.1102230246251565e-16 * 3]])
+    x = x[x > 0.0]
+    y = G.math_probe("log_unit", x)
+    ref = correctly_rounded(mp.log, x)
+    err = ulp_err(y, ref)
+    assert err[x != 1.0].max() <= 2.0, (x[np.argmax(err)], err.max())
+    assert y[x == 1.0].max() == 0.0
+
+
+def test_sincos_2pi_absolute():
+    """sin and cos of 2 pi u on [0, 1), the Box-Muller angle: within 2e-16 of
+    the exact value (the argument's own rounding bounds the rest)."""
+    rng = np.random.default_rng(14)
+    x = np.concatenate([rng.uniform(0.0, 1.0, 2000), np.arange(0, 64) / 64.0,
+                        [np.nextafter(1.0, 0.0), 2.0 ** -53, 0.125, 0.375, 0.625, 0.875]])
+    mp.mp.dps = 40
+    ref_s = np.array([float(mp.sin(2 * mp.pi * mp.mpf(float(v)))) for v in x])
+    ref_c = np.array([float(mp.cos(2 * mp.pi * mp.mpf(float(v)))) for v in x])
+    s = G.math_probe("sin_2pi", x)
+    c = G.math_probe("cos_2pi", x)
+    assert np.max(np.abs(s - ref_s)) <= 2e-16
+    assert np.max(np.abs(c - ref_c)) <= 2e-16
+    assert np.max(np.abs(s * s + c * c - 1.0)) <= 4e-16
