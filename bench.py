@@ -236,8 +236,10 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic_from_profile(),
                     compute="FP64: pairs, exchange and springs on the VALU, the normal-mode transforms "
-                            "(19% of the flops) on the matrix cores (v_mfma_f64_16x16x4f64); MI355X's FP64 "
-                            "vector peak equals its FP64 matrix peak (78.6 TFLOP/s)",
+                            "(19% of the counted flops) on the matrix cores (v_mfma_f64_16x16x4f64), of which "
+                            "the kernel performs half (the O-step transforms only the noise: C^T (c1 C p + "
+                            "c2 xi) = c1 p + C^T c2 xi); MI355X's FP64 vector peak equals its FP64 matrix "
+                            "peak (78.6 TFLOP/s)",
                     kernel="k_batch_step<3,AZIZ,STEP> (persistent, %d replicas)" % R, ms_per_launch=ms_launch,
                     timing="hipEvents on the batch stream around the timed steps",
                     flops_per_launch=fl, exps_per_launch=xp, frac_with_exps=t_roof / (ms_launch * 1e-3))

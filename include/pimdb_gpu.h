@@ -194,7 +194,10 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out);
 void pimdb_batch_destroy(pimdb_batch* batch);
 /* Last failure of batch; with batch == NULL, the last failed create call. */
 const char* pimdb_batch_last_error(const pimdb_batch* batch);
-/* Per-replica state, host layout [bead][particle][axis]. */
+/* Per-replica state, host layout [bead][particle][axis].  With fixcom and a
+ * normal-mode Langevin thermostat on the matrix-core path (nbeads <= 64), the
+ * COM subtraction after a step's last O-step is left pending on the device
+ * and applied by the next step; get_state returns p with it applied. */
 int pimdb_batch_set_state(pimdb_batch* batch, int rep, const double* x, const double* p);
 int pimdb_batch_get_state(pimdb_batch* batch, int rep, double* x, double* p, double* f);
 /* Forces of every replica (pimdb_forces for each). */

@@ -125,7 +125,8 @@ def test_batch_follows_single_system(which):
 @pytest.mark.parametrize("which", ["dipole", "aziz"])
 def test_batch_graph_replays_are_deterministic(which):
     """Two batches of the same replicas agree bitwise after every replay
-    (the Aziz case exercises the deferred damping queue's accumulation)."""
+    (the Aziz case exercises the LDS force accumulation of the pair phase and
+    the exchange's chunked exterior forces)."""
     reps = dipole_replicas(9, seed=8) if which == "dipole" else aziz_replicas(4)
     bs = [G.Batch([G.make_params(**kw) for kw, _, _ in reps]) for _ in range(2)]
     for b in bs:
