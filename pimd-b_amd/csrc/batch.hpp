@@ -27,7 +27,7 @@ struct RepDesc {
     int fused;                  // normal-mode O-step on the matrix cores with B + A fused in (see wg_momenta)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
-    double int_k, strength, cutoff;
+    double int_k, strength, cutoff, cut2;
     ExtPot ext;
     uint32_t seed, rid;
     unsigned long long off_x;   // P*D*N block of x, p, f
@@ -89,7 +89,7 @@ __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     u.plds = uni(r.plds); u.fused = uni(r.fused);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
-    u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff);
+    u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff); u.cut2 = uni(r.cut2);
     u.ext.kind = uni(r.ext.kind); u.ext.a = uni(r.ext.a); u.ext.b = uni(r.ext.b); u.ext.c = uni(r.ext.c);
     u.ext.d = uni(r.ext.d);
     u.seed = (uint32_t)uni((int)r.seed); u.rid = (uint32_t)uni((int)r.rid);
@@ -518,7 +518,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     const size_t bs = (size_t)D * N;
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
-    const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.int_k, r.strength);
+    const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.cut2, r.int_k, r.strength);
     lds_double* Fw = F + (size_t)wave * bs;   // this wave's slice forces (r.plds)
     for (int b = wave; b < P; b += nw) {
         const double* xs = x + bs * b;

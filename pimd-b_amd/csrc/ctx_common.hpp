@@ -162,6 +162,21 @@ inline void from_soa(const double* s, double* h, int N, int P, int D) {
             for (int a = 0; a < D; ++a) h[((size_t)b * N + i) * D + a] = s[((size_t)b * D + a) * N + i];
 }
 
+// The smallest double t with sqrt(t) >= c (IEEE sqrt, correctly rounded), so
+// that r2 < t exactly when sqrt(r2) < c: the reference's strict cutoff test
+// on the norm (simulation.cpp:444) as one comparison of the squared distance.
+inline double cutoff_threshold(double c) {
+    if (!(c > 0.0)) return 0.0;
+    double t = c * c;
+    while (std::sqrt(t) < c) t = std::nextafter(t, INFINITY);
+    for (;;) {
+        const double lower = std::nextafter(t, 0.0);
+        if (std::sqrt(lower) >= c) t = lower;
+        else break;
+    }
+    return t;
+}
+
 inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     ForceArgs fa{};
     fa.N = p.natoms; fa.P = p.nbeads; fa.bosonic = d.bosonic ? 1 : 0; fa.pbc = p.pbc;
@@ -169,6 +184,7 @@ inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     fa.ext = d.ext;
     fa.int_k = d.int_k; fa.strength = p.int_strength;
     fa.cutoff = d.cutoff;
+    fa.cut2 = cutoff_threshold(d.cutoff);
     return fa;
 }
 

@@ -25,6 +25,7 @@ struct ForceArgs {
     ExtPot ext;              // external potential
     double int_k, strength;  // pair harmonic constant, dipole strength
     double cutoff;           // effective cutoff (0: no pair loop; < 0: all pairs)
+    double cut2;             // cutoff_threshold(cutoff)
 };
 
 constexpr int FORCE_TILE = 64;
@@ -56,7 +57,7 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
             for (int a = 0; a < D; ++a) xi[a] = sx[a * N + i];
             const int chunk = (N + FORCE_WAVES - 1) / FORCE_WAVES;
             const int jlo = wave * chunk, jhi = min(N, jlo + chunk);
-            const PairParams pp = make_pair_params(fa.L, fa.pbc, fa.cutoff, fa.int_k, fa.strength);
+            const PairParams pp = make_pair_params(fa.L, fa.pbc, fa.cutoff, fa.cut2, fa.int_k, fa.strength);
             auto run = [&](auto img) {
                 constexpr int IMG = decltype(img)::value;
                 for (int j = jlo; j < jhi; ++j) {

@@ -26,20 +26,20 @@ constexpr int PTILE = 64;
 
 struct PairParams {
     double L, invL;
-    double cutoff, cut2_lo, cut2_hi;   // c^2 (1 -/+ 1e-12): outside this band r^2 decides alone
+    double cutoff;
+    double cut2;                       // r^2 < cut2  <=>  sqrt(r^2) < cutoff (cutoff_threshold)
     double int_k, strength;
     int pbc, guard;                    // guard: exact image choice near L/2 (PBC without a cutoff)
 };
 
-__host__ __device__ inline PairParams make_pair_params(double L, int pbc, double cutoff, double int_k,
+__host__ __device__ inline PairParams make_pair_params(double L, int pbc, double cutoff, double cut2, double int_k,
                                                        double strength) {
     PairParams pp;
     pp.L = L;
     pp.invL = 1.0 / L;
     pp.pbc = pbc;
     pp.cutoff = cutoff;
-    pp.cut2_lo = cutoff * cutoff * (1.0 - 1e-12);
-    pp.cut2_hi = cutoff * cutoff * (1.0 + 1e-12);
+    pp.cut2 = cut2;
     pp.int_k = int_k;
     pp.strength = strength;
     pp.guard = (pbc && cutoff < 0.0) ? 1 : 0;
@@ -71,11 +71,11 @@ __device__ __forceinline__ double rsqrt_refined(double r2) {
     return fma(y0 * e, fma(e, 0.375, 0.5), y0);
 }
 
-// The reference's strict test |d| < cutoff (simulation.cpp:444) for the rare
-// r^2 within 1e-12 of cutoff^2, where r^2 alone cannot decide; |d| is sqrt of
-// the same sum of squares the reference takes the norm of.  Out of line so
-// the compiler does not evaluate the square root for every pair.
-static __device__ __noinline__ bool within_cutoff_exact(double r2, double cutoff) { return sqrt(r2) < cutoff; }
+// The reference's strict test |d| < cutoff (simulation.cpp:444) on a squared
+// distance r2: the square root is correctly rounded and increasing, so
+// sqrt(r2) < cutoff exactly when r2 is below the smallest double whose root
+// reaches the cutoff.  That threshold is found once on the host
+// (cutoff_threshold, ctx_common.hpp); the pair loop compares r2 with it.
 
 // gradV(r_i - r_j) = g d for the pair (i, j): d = r_i - r_j (min-imaged) and
 // g, if the pair is inside the cutoff (distance < cutoff, or cutoff < 0);
@@ -86,9 +86,7 @@ __device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&
 #pragma unroll
     for (int a = 0; a < D; ++a) d[a] = min_image_pair<IMG>(xi[a] - xj[a], pp);
     const double r2 = norm2_fma<D>(d);
-    bool in = pp.cutoff < 0.0 || r2 < pp.cut2_lo;
-    if (!in && r2 < pp.cut2_hi) in = within_cutoff_exact(r2, pp.cutoff);
-    if (!in) return false;
+    if (!(pp.cutoff < 0.0 || r2 < pp.cut2)) return false;
     const double inv = rsqrt_refined(r2);
     g = pair_grad_factor_fast<KIND>(r2 * inv, inv, pp.int_k, pp.strength);
     return true;
@@ -97,6 +95,10 @@ __device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&
 // LDS pointer type: accesses through it are ds_ instructions that the
 // compiler knows cannot alias global memory.
 typedef __attribute__((address_space(3))) double lds_double;
+// Global-memory pointer type: loads through it are global_load (counted by
+// vmcnt only), not flat loads, whose lgkmcnt share would make the pair loop
+// wait for its own LDS force updates on every step.
+typedef const __attribute__((address_space(1))) double* gbl_cdptr;
 
 // All pairs of one slice on one wave.  xs: slice positions [D][N]; fo: the
 // slice's force array [D][N], owned by this wave, overwritten with the pair
@@ -167,8 +169,9 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
 // caller adds the rest of the force and writes the slice once).  Used when
 // every wave's slice fits the LDS.
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ void slice_pair_forces_lds(const double* xs, int N, const PairParams& pp, int lane,
+__device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const PairParams& pp, int lane,
                                                       lds_double* F) {
+    const gbl_cdptr xs = (gbl_cdptr)xs_;
     for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
     __builtin_amdgcn_wave_barrier();
     const int T = (N + PTILE - 1) / PTILE;
@@ -184,25 +187,27 @@ __device__ __forceinline__ void slice_pair_forces_lds(const double* xs, int N, c
         for (int J = I; J < T; ++J) {
             const bool diag = J == I;
             const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
-            int jc = J * PTILE + ((lane + s0) & (PTILE - 1));
+            const int jc0 = J * PTILE + ((lane + s0) & (PTILE - 1));
+            // the column's index, as its first element in F (jd = jc D), moves with it
+            unsigned jd = (unsigned)(jc0 * D);
             double xj[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) xj[a] = jc < N ? xs[a * N + jc] : 0.0;
+            for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
             for (int s = s0; s < s1; ++s) {
-                if (row_ok && jc < N && (!diag || s < 32 || lane < 32)) {
+                if (row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32)) {
                     double d[D], g;
                     if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
+                        lds_double* Fj = F + jd;
 #pragma unroll
                         for (int a = 0; a < D; ++a) {
                             racc[a] = fma(-g, d[a], racc[a]);
-                            __hip_atomic_fetch_add(&F[jc * D + a], g * d[a], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&Fj[a], g * d[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
                 }
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
-                jc = lane_rotate(jc);
+                jd = (unsigned)lane_rotate((int)jd);
             }
         }
         if (row_ok) {

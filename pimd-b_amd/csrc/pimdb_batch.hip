@@ -216,6 +216,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
         d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
         d.ext = dv.ext; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
+        d.cut2 = cutoff_threshold(dv.cutoff);
         d.seed = p.seed; d.rid = 0;   // noise stream keyed by the seed, as in the single-system form
         d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
         d.off_V = b->nV; b->nV += (size_t)p.natoms + 1;
