@@ -372,6 +372,18 @@ __device__ __forceinline__ double rec_inblock_seq(const double* a, int t, double
     return v;
 }
 
+// a[j][t] = v for j >= t (the upper triangle and the diagonal): the strict
+// upper triangle is the transpose of rec_tri's, the diagonal one entry per
+// lane.  Run by the group's nw waves.
+__device__ __forceinline__ void rec_fill_upper(double* a, int wave, int lane, int nw, double v) {
+    for (int vr = wave; vr < REC_B / 2; vr += nw) {
+        int t, j;
+        rec_tri(vr, lane, t, j);
+        if (t >= 0) a[t * REC_LD + j] = v;
+    }
+    if (wave == 0) a[lane * REC_LD + lane] = v;
+}
+
 // Blocks of the recursions for N particles; the backward recursion has N-1
 // targets and may leave the last block empty.
 __host__ __device__ inline int rec_blocks(int N) { return (N + REC_B - 1) / REC_B; }
@@ -420,12 +432,11 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             if (t >= 0 && t < nb) {
                 const RecTarget<D> tt = rec_target<DIR, D>(sh, N, n0 + t);
                 sh.a[j * REC_LD + t] = rec_w<DIR, D>(sh, r, bt, N, tt, n0 + j);
+            } else if (t >= nb) {
+                sh.a[j * REC_LD + t] = INFINITY;
             }
         }
-        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
-            const int j = idx / REC_B, t = idx % REC_B;
-            if (j >= t || t >= nb) sh.a[j * REC_LD + t] = INFINITY;
-        }
+        rec_fill_upper(sh.a, wave, lane, nthr / WAVE, INFINITY);
         // (1) one pass over this wave's share of earlier sources: the lane's
         // running minimum m of y_q = w(n,q) + U_q and s = sum_q exp(-(y_q - m)),
         // rescaled whenever a chunk lowers m
@@ -484,11 +495,9 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             int t, j;
             rec_tri(vr, lane, t, j);
             if (t >= 0 && t < nb) sh.a[j * REC_LD + t] = exp_minimax(-(sh.a[j * REC_LD + t] + sh.R[j] - sh.R[t]));
+            else if (t >= nb) sh.a[j * REC_LD + t] = 0.0;
         }
-        for (int idx = tid; idx < REC_B * REC_B; idx += nthr) {
-            const int j = idx / REC_B, t = idx % REC_B;
-            if (j >= t || t >= nb) sh.a[j * REC_LD + t] = 0.0;
-        }
+        rec_fill_upper(sh.a, wave, lane, nthr / WAVE, 0.0);
         __syncthreads();
         // (2c) in-block forward substitution on wave 0 (a[j][t] = 0 for
         // j >= t), then U = R - ln z
