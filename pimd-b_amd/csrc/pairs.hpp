@@ -159,15 +159,79 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
     }
 }
 
-// The same pairs with the slice's forces accumulated in LDS: F is this
-// wave's [N][D] array in LDS (particle-major, so a particle's axes sit at
-// constant offsets from one address).  The column particles' forces are added to F at
-// every step (a no-return LDS atomic add; the 64 lanes of a step hold 64
-// distinct columns, so the adds do not collide and land in step order), so
-// only the positions and the column index rotate along the wave.  The row
-// forces are added at the end of each row tile; the result stays in F (the
-// caller adds the rest of the force and writes the slice once).  Used when
-// every wave's slice fits the LDS.
+// All pairs of one slice on one wave, with the column forces accumulated in
+// the wave's own LDS array F[N][D] (zeroed here; the pair forces are left in
+// it: the caller adds the rest of the force and writes the slice once).  Used
+// when every wave's slice fits the LDS.
+//
+// Row tiles are taken two at a time where they can be: for tile pairs (I, J)
+// and (I+1, J) with J >= I+2, lane l holds rows 64I+l and 64(I+1)+l and one
+// rotating column of tile J, so the column's DPP rotation and its LDS update
+// (the two pairs' terms added first) serve two pairs.  The tiles (I, I),
+// (I, I+1), (I+1, I+1) and an odd last row tile take one row at a time.
+template <int D, int KIND, int IMG>
+__device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
+                                               bool diag, const double (&xi)[D], bool row_ok, double (&racc)[D],
+                                               lds_double* F) {
+    const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
+    const int jc0 = J * PTILE + ((lane + s0) & (PTILE - 1));
+    // the column's index, as its first element in F (jd = jc D), moves with it
+    unsigned jd = (unsigned)(jc0 * D);
+    double xj[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
+    for (int s = s0; s < s1; ++s) {
+        if (row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32)) {
+            double d[D], g;
+            if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
+                lds_double* Fj = F + jd;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    racc[a] = fma(-g, d[a], racc[a]);
+                    __hip_atomic_fetch_add(&Fj[a], g * d[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
+        jd = (unsigned)lane_rotate((int)jd);
+    }
+}
+
+// Rows 64I+l and 64(I+1)+l (both present) against the full column tile J.
+template <int D, int KIND, int IMG>
+__device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
+                                                   const double (&xa)[D], const double (&xb)[D], double (&ra)[D],
+                                                   double (&rb)[D], lds_double* F) {
+    const int jc0 = J * PTILE + lane;
+    unsigned jd = (unsigned)(jc0 * D);
+    double xj[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
+    for (int s = 0; s < PTILE; ++s) {
+        if (jd < (unsigned)(N * D)) {
+            double da[D], db[D], ga = 0.0, gb = 0.0;
+            const bool ina = pair_grad<D, KIND, IMG>(xa, xj, pp, da, ga);
+            const bool inb = pair_grad<D, KIND, IMG>(xb, xj, pp, db, gb);
+            if (ina || inb) {
+                if (!ina) ga = 0.0;
+                if (!inb) gb = 0.0;
+                lds_double* Fj = F + jd;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    ra[a] = fma(-ga, da[a], ra[a]);
+                    rb[a] = fma(-gb, db[a], rb[a]);
+                    __hip_atomic_fetch_add(&Fj[a], fma(ga, da[a], gb * db[a]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
+        jd = (unsigned)lane_rotate((int)jd);
+    }
+}
+
 template <int D, int KIND, int IMG>
 __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const PairParams& pp, int lane,
                                                       lds_double* F) {
@@ -175,45 +239,34 @@ __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, 
     for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
     __builtin_amdgcn_wave_barrier();
     const int T = (N + PTILE - 1) / PTILE;
-    for (int I = 0; I < T; ++I) {
-        const int i = I * PTILE + lane;
-        const bool row_ok = i < N;
-        double xi[D], racc[D];
+    for (int I = 0; I < T; I += 2) {
+        const bool two = I + 1 < T;
+        const int ia = I * PTILE + lane, ib = (I + 1) * PTILE + lane;
+        const bool oka = ia < N, okb = two && ib < N;
+        double xa[D], xb[D], ra[D], rb[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            xi[a] = row_ok ? xs[a * N + i] : 0.0;
-            racc[a] = 0.0;
+            xa[a] = oka ? xs[a * N + ia] : 0.0;
+            xb[a] = okb ? xs[a * N + ib] : 0.0;
+            ra[a] = 0.0;
+            rb[a] = 0.0;
         }
-        for (int J = I; J < T; ++J) {
-            const bool diag = J == I;
-            const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
-            const int jc0 = J * PTILE + ((lane + s0) & (PTILE - 1));
-            // the column's index, as its first element in F (jd = jc D), moves with it
-            unsigned jd = (unsigned)(jc0 * D);
-            double xj[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
-            for (int s = s0; s < s1; ++s) {
-                if (row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32)) {
-                    double d[D], g;
-                    if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
-                        lds_double* Fj = F + jd;
-#pragma unroll
-                        for (int a = 0; a < D; ++a) {
-                            racc[a] = fma(-g, d[a], racc[a]);
-                            __hip_atomic_fetch_add(&Fj[a], g * d[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
-                jd = (unsigned)lane_rotate((int)jd);
-            }
+        pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I, true, xa, oka, ra, F);
+        if (two) {
+            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, false, xa, oka, ra, F);
+            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, true, xb, okb, rb, F);
+            // a tile J >= I+2 exists only when tiles I and I+1 are full
+            for (int J = I + 2; J < T; ++J) pair_tile_two_rows<D, KIND, IMG>(xs, N, pp, lane, J, xa, xb, ra, rb, F);
         }
-        if (row_ok) {
+        if (oka) {
 #pragma unroll
             for (int a = 0; a < D; ++a)
-                __hip_atomic_fetch_add(&F[i * D + a], racc[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&F[ia * D + a], ra[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (okb) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                __hip_atomic_fetch_add(&F[ib * D + a], rb[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     __builtin_amdgcn_wave_barrier();
