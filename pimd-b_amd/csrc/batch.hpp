@@ -120,11 +120,11 @@ __host__ __device__ inline int nm_mfma_groups(int P, int nwaves) {
     const int mb = nm_pad16(P) / 16;
     return nwaves >= mb ? nwaves / mb : 1;
 }
-// LDS doubles of the matrix-core path: C^T padded to Pp x Pp and the noise
-// tile [groups][Pp][16].
+// LDS doubles of the matrix-core path: C^T padded to Pp x Pp and two noise
+// tiles [groups][Pp][16].
 __host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
     const size_t Pp = (size_t)nm_pad16(P);
-    return Pp * Pp + (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
+    return Pp * Pp + 2 * (size_t)nm_mfma_groups(P, nwaves) * Pp * 16;
 }
 
 // Element loops over a replica's [bead][axis][particle] arrays take
@@ -244,31 +244,64 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         double* x = uni(bp.x) + r.off_x;
         const int q = lane / 16, c16 = lane % 16;
         const int half = (P + 1) / 2, width = G * 16, npair = (Pp / 2) * width;
-        for (int cb0 = 0; cb0 < NCB; cb0 += G) {
-            const int c0 = cb0 * 16;
+        // c2 xi of the round starting at column block cb into buf (every thread its share)
+        auto gen_noise = [&](int cb, lds_double* buf) {
+            const int c0n = cb * 16;
             for (int e = threadIdx.x; e < npair; e += blockDim.x) {
-                const int m = e / width, gc = e % width, col = c0 + gc;
+                const int m = e / width, gc = e % width, col = c0n + gc;
                 double g0 = 0.0, g1 = 0.0;
                 if (m < half && col < ncols)
                     gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
-                lds_double* out = pn + (size_t)(gc / 16) * Pp * 16 + gc % 16;
+                lds_double* out = buf + (size_t)(gc / 16) * Pp * 16 + gc % 16;
                 out[(2 * m) * 16] = 2 * m < P ? r.c2 * g0 : 0.0;
                 out[(2 * m + 1) * 16] = 2 * m + 1 < P ? r.c2 * g1 : 0.0;
             }
-            __syncthreads();
+        };
+        // two noise tiles: a round's products read one while the next round's
+        // noise is drawn into the other, between the epilogue's loads and its
+        // arithmetic, so the draws overlap the memory traffic
+        lds_double* pnb[2] = {pn, pn + (size_t)G * Pp * 16};
+        gen_noise(0, pnb[0]);
+        __syncthreads();
+        for (int cb0 = 0, rnd = 0; cb0 < NCB; cb0 += G, ++rnd) {
+            const int c0 = cb0 * 16;
+            const lds_double* cur = pnb[rnd & 1];
+            lds_double* nxt = pnb[(rnd + 1) & 1];
+            const bool more = cb0 + G < NCB;
+            bool drawn = false;
             for (int task = wave; task < ntask; task += nw) {
                 const int bb = task % MB, g = task / MB;
-                if (cb0 + g >= NCB) continue;
-                const lds_double* B = pn + (size_t)g * Pp * 16;
+                const int col = c0 + g * 16 + c16;
+                const bool live = cb0 + g < NCB;   // wave-uniform
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
-                for (int kk = 0; kk < KK; ++kk) {
-                    const int k = 4 * kk + q;
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CTl[(size_t)k * Pp + bb * 16 + c16],
-                                                                B[k * 16 + c16], acc, 0, 0, 0);
+                double pv[4], fv[4], xv[4];
+                if (live) {
+                    const lds_double* B = cur + (size_t)g * Pp * 16;
+                    for (int kk = 0; kk < KK; ++kk) {
+                        const int k = 4 * kk + q;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CTl[(size_t)k * Pp + bb * 16 + c16],
+                                                                    B[k * 16 + c16], acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int b = bb * 16 + q + 4 * v;
+                        pv[v] = 0.0;
+                        fv[v] = 0.0;
+                        xv[v] = 0.0;
+                        if (b < P && col < ncols) {
+                            const size_t e = (size_t)b * ncols + col;
+                            pv[v] = p[e];
+                            if (kick || drift) fv[v] = f[e];
+                            if (drift) xv[v] = x[e];
+                        }
+                    }
+                }
+                if (!drawn) {
+                    if (more) gen_noise(cb0 + G, nxt);
+                    drawn = true;
                 }
                 // acc[v] = (C^T c2 xi) of bead 16bb + q + 4v, column c0 + 16g + c16
-                const int col = c0 + g * 16 + c16;
-                if (col < ncols) {
+                if (live && col < ncols) {
                     const int ax = col / N;
                     double cmi = 0.0, cmo = 0.0;
 #pragma unroll
@@ -276,20 +309,6 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                         if (a == ax) {
                             cmi = cmin[a];
                             cmo = cm1[a];
-                        }
-                    }
-                    double pv[4], fv[4], xv[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int b = bb * 16 + q + 4 * v;
-                        pv[v] = 0.0;
-                        fv[v] = 0.0;
-                        xv[v] = 0.0;
-                        if (b < P) {
-                            const size_t e = (size_t)b * ncols + col;
-                            pv[v] = p[e];
-                            if (kick || drift) fv[v] = f[e];
-                            if (drift) xv[v] = x[e];
                         }
                     }
 #pragma unroll
@@ -315,7 +334,8 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                     }
                 }
             }
-            __syncthreads();   // pn is rewritten by the next round
+            if (!drawn && more) gen_noise(cb0 + G, nxt);
+            __syncthreads();   // the next round reads nxt; cur is redrawn the round after
         }
     } else if (r.thermostat != 1 || !r.nm) {
         // beads 2m and 2m+1 of a column share one Box-Muller pair
