@@ -403,7 +403,8 @@ template <int DIR, int D, int UNROLL = REC_UNROLL>
 __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __restrict__ V, double* __restrict__ Vb,
                                    int* __restrict__ status, RecShared sh, int gtid, int gnthr) {
     const int tid = gtid, nthr = gnthr;
-    const int lane = tid % WAVE, wave = tid / WAVE;
+    // the wave index in a scalar register: loops over it are uniform
+    const int lane = tid % WAVE, wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
     const int nwave = min(nthr / WAVE, REC_MAXWAVES);
     const int ntarget = DIR == 0 ? N : N - 1;
     for (int i = tid; i <= N; i += nthr) {
@@ -688,7 +689,7 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                                                    const double* __restrict__ x2, const double* __restrict__ xPm1,
                                                    double* __restrict__ f1, double* __restrict__ fP, double* scratch) {
     const int nthr = blockDim.x, tid = threadIdx.x;
-    const int lane = tid % WAVE, wave = tid / WAVE, nw = nthr / WAVE;
+    const int lane = tid % WAVE, wave = __builtin_amdgcn_readfirstlane(tid / WAVE), nw = nthr / WAVE;
     static_assert(D + 4 <= EXT_REC, "column record");
     double* Vl = scratch;                      // [N+1] V
     double* Vbn = Vl + N + 1;                  // [N]   Vb[b+1] - V[N]
