@@ -191,8 +191,12 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     const double half_dt = 0.5 * r.dt;
     const int ncols = D * N;
     double csum[D];   // this thread's share of the momentum sums per axis
+    double fsum[D];   // fused first O-step: the sum of f per axis
 #pragma unroll
-    for (int a = 0; a < D; ++a) csum[a] = 0.0;
+    for (int a = 0; a < D; ++a) {
+        csum[a] = 0.0;
+        fsum[a] = 0.0;
+    }
     if (r.thermostat == 1 && r.nm && P <= NM_MFMA_MAXP) {
         // Matrix-core form.  One friction serves every mode (langevin.cpp:15-27,
         // thermostat_coupling.cpp:27-47), and C is orthogonal, so the O-step
@@ -215,30 +219,64 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             const int a = e / Pp, b = e % Pp;
             CTl[e] = (a < P && b < P) ? CTt[a * P + b] : 0.0;
         }
-        // first O-step: the pending COM and the COM of the result
-        double cmin[D], cm1[D];
+        // first O-step of a fused replica.  When p is pending (cmp[7]: it
+        // holds the last step's p after its first B, the second B and O-step
+        // still owed), both O-steps run here in one product:
+        //   p2 = c1 (p + dt/2 f) + C^T c2 xi2 - cm2,   p1 = c1 p2 + C^T c2 xi1
+        //      = c1 (c1 (p + dt/2 f) - cm2) + C^T c2 (c1 xi2 + xi1),
+        // xi2 the last step's second O-step noise (keys: this step, O-step 1),
+        // xi1 this step's first.  COMs as above, with the sum of p + dt/2 f
+        // from the known sum of p (cmp[4..]) and the sum of f.
+        double cmin[D], cm1[D], cm2[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             cmin[a] = 0.0;
             cm1[a] = 0.0;
+            cm2[a] = 0.0;
         }
+        const bool pend = drift && uni(cmp[7]) != 0.0;
         if (drift && r.fixcom) {
-            double s0[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) s0[a] = 0.0;
-            for (int col = threadIdx.x; col < ncols; col += blockDim.x) {
-                double g0, g1;
-                gauss2((uint32_t)col, step, ostep, r.rid, r.seed, g0, g1);   // mode 0 of column col
-                const int ax = col / N;
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-                    if (a == ax) s0[a] += g0;
-            }
-            const double sq = sqrt((double)P);
+            double s1[D], s2[D], sf[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                cmin[a] = uni(cmp[a]);
-                cm1[a] = (r.c1 * uni(cmp[4 + a]) + r.c2 * sq * wg_sum(s0[a], red)) / (double)(N * P);
+                s1[a] = 0.0;
+                s2[a] = 0.0;
+                sf[a] = 0.0;
+            }
+            for (int col = threadIdx.x; col < ncols; col += blockDim.x) {
+                double g0, g1, h0 = 0.0, h1;
+                gauss2((uint32_t)col, step, ostep, r.rid, r.seed, g0, g1);   // mode 0 of column col
+                if (pend) gauss2((uint32_t)col, step, 1u, r.rid, r.seed, h0, h1);
+                const int ax = col / N;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    if (a == ax) {
+                        s1[a] += g0;
+                        s2[a] += h0;
+                    }
+                }
+            }
+            const int nel = P * ncols;
+            for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+                const double fe = f[e];
+                const int ax = elem_axis<D>(e, N);
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+                    if (a == ax) sf[a] += fe;
+            }
+            const double sq = sqrt((double)P), inv = 1.0 / (double)(N * P);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const double S1 = wg_sum(s1[a], red);
+                fsum[a] = wg_sum(sf[a], red);
+                if (pend) {
+                    const double S2 = wg_sum(s2[a], red);
+                    cm2[a] = (r.c1 * (uni(cmp[4 + a]) + half_dt * fsum[a]) + r.c2 * sq * S2) * inv;
+                    cm1[a] = r.c2 * sq * S1 * inv;   // p2 sums to zero
+                } else {
+                    cmin[a] = uni(cmp[a]);
+                    cm1[a] = (r.c1 * uni(cmp[4 + a]) + r.c2 * sq * S1) * inv;
+                }
             }
         }
         double* x = uni(bp.x) + r.off_x;
@@ -250,8 +288,15 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             for (int e = threadIdx.x; e < npair; e += blockDim.x) {
                 const int m = e / width, gc = e % width, col = c0n + gc;
                 double g0 = 0.0, g1 = 0.0;
-                if (m < half && col < ncols)
+                if (m < half && col < ncols) {
                     gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                    if (pend) {   // c1 xi2 + xi1
+                        double h0, h1;
+                        gauss2((uint32_t)((size_t)m * ncols + col), step, 1u, r.rid, r.seed, h0, h1);
+                        g0 = fma(r.c1, h0, g0);
+                        g1 = fma(r.c1, h1, g1);
+                    }
+                }
                 lds_double* out = buf + (size_t)(gc / 16) * Pp * 16 + gc % 16;
                 out[(2 * m) * 16] = 2 * m < P ? r.c2 * g0 : 0.0;
                 out[(2 * m + 1) * 16] = 2 * m + 1 < P ? r.c2 * g1 : 0.0;
@@ -260,13 +305,14 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         // two noise tiles: a round's products read one while the next round's
         // noise is drawn into the other, between the epilogue's loads and its
         // arithmetic, so the draws overlap the memory traffic
-        lds_double* pnb[2] = {pn, pn + (size_t)G * Pp * 16};
-        gen_noise(0, pnb[0]);
+        lds_double* const pn0 = pn;
+        lds_double* const pn1 = pn + (size_t)G * Pp * 16;
+        gen_noise(0, pn0);
         __syncthreads();
         for (int cb0 = 0, rnd = 0; cb0 < NCB; cb0 += G, ++rnd) {
             const int c0 = cb0 * 16;
-            const lds_double* cur = pnb[rnd & 1];
-            lds_double* nxt = pnb[(rnd + 1) & 1];
+            const lds_double* cur = (rnd & 1) ? pn1 : pn0;
+            lds_double* nxt = (rnd & 1) ? pn0 : pn1;
             const bool more = cb0 + G < NCB;
             bool drawn = false;
             for (int task = wave; task < ntask; task += nw) {
@@ -303,12 +349,13 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                 // acc[v] = (C^T c2 xi) of bead 16bb + q + 4v, column c0 + 16g + c16
                 if (live && col < ncols) {
                     const int ax = col / N;
-                    double cmi = 0.0, cmo = 0.0;
+                    double cmi = 0.0, cmo = 0.0, cmt = 0.0;
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
                         if (a == ax) {
                             cmi = cmin[a];
                             cmo = cm1[a];
+                            cmt = cm2[a];
                         }
                     }
 #pragma unroll
@@ -317,7 +364,8 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                         if (b < P) {
                             const size_t e = (size_t)b * ncols + col;
                             if (drift) {
-                                const double pnew = r.c1 * (pv[v] - cmi) + acc[v];
+                                const double pnew = pend ? r.c1 * (r.c1 * (pv[v] + half_dt * fv[v]) - cmt) + acc[v]
+                                                         : r.c1 * (pv[v] - cmi) + acc[v];
                                 double w = pnew - cmo;
                                 w = w + half_dt * fv[v];
                                 p[e] = w;
@@ -459,9 +507,21 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             __syncthreads();
         }
     }
-    if (r.fixcom && drift) {
+    if (drift) {
+        // p now holds this step's p after its first B: the second B and
+        // O-step are owed (done by the next step's first O-step, or by
+        // pimdb_batch_get_state's finishing launch); p sums to dt/2 sum f
 #pragma unroll
         for (int a = 0; a < D; ++a) cm[a] = 0.0;   // removed in the epilogue
+        __syncthreads();   // every thread has read cmp
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                cmp[a] = 0.0;
+                cmp[4 + a] = half_dt * fsum[a];
+            }
+            cmp[7] = 1.0;
+        }
     } else if (r.fixcom) {
 #pragma unroll
         for (int a = 0; a < D; ++a) cm[a] = wg_sum(csum[a], red) / (double)(N * P);
@@ -472,6 +532,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                     cmp[a] = cm[a];
                     cmp[4 + a] = 0.0;
                 }
+                cmp[7] = 0.0;
             }
         } else if (subtract) {
             __syncthreads();
@@ -480,6 +541,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     } else {
 #pragma unroll
         for (int a = 0; a < D; ++a) cm[a] = 0.0;
+        if (r.fused && threadIdx.x == 0) cmp[7] = 0.0;   // the owed O-step is done
     }
     __syncthreads();
 }
@@ -640,11 +702,12 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
 }
 
 // Work of one launch: a full MD step, or parts of it for timing.
-enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3, BM_INTEGRATOR = 4 };
+enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3, BM_INTEGRATOR = 4, BM_FINISH = 5 };
 
-// One MD step of every replica (BM_STEP), the forces only (BM_FORCES), or one
-// phase for timing.  The mode is a template parameter so that every mode is
-// its own kernel (and its own line in a kernel trace).
+// One MD step of every replica (BM_STEP), the forces only (BM_FORCES), one
+// phase for timing, or the owed second B and O-step of replicas whose p is
+// pending (BM_FINISH; see wg_momenta).  The mode is a template parameter so
+// that every mode is its own kernel (and its own line in a kernel trace).
 template <int D, int KIND, int MODE>
 static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
     extern __shared__ double smem[];
@@ -666,8 +729,18 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
                            bp.Vb + r.off_V, bp.status + ri, lds, r.xdual);
         if constexpr (integ) {
-            double cm[D];
-            wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
+            if (!r.fused) {   // fused replicas owe it to the next step's first O-step
+                double cm[D];
+                wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
+            }
+        }
+        if constexpr (MODE == BM_FINISH) {
+            // the O-step keys of the step that left p pending: its step + 1,
+            // which is the counter now
+            if (r.fused && bp.cmp[8 * (size_t)ri + 7] != 0.0) {
+                double cm[D];
+                wg_momenta<D>(r, bp, true, step, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
+            }
         }
     }
     if constexpr (MODE == BM_STEP) {

@@ -33,6 +33,8 @@ struct pimdb_batch {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    bool fused_any = false;       // some replica takes the fused normal-mode path
+    bool maybe_pending = false;   // a step ran since the last finishing launch
     std::string err;
     int fail(int code, const std::string& m) {
         err = m;
@@ -105,6 +107,7 @@ static void launch_batch_dk(pimdb_batch* b, int mode) {
     case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES>), grid, block, b->lds, b->s, b->bp); break;
     case BM_PAIRS: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_PAIRS>), grid, block, b->lds, b->s, b->bp); break;
     case BM_EXCHANGE: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_EXCHANGE>), grid, block, b->lds, b->s, b->bp); break;
+    case BM_FINISH: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FINISH>), grid, block, b->lds, b->s, b->bp); break;
     default: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_INTEGRATOR>), grid, block, b->lds, b->s, b->bp); break;
     }
 }
@@ -213,6 +216,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         d.N = p.natoms; d.P = p.nbeads; d.bosonic = dv.bosonic; d.pbc = p.pbc;
         d.thermostat = p.thermostat; d.nm = p.nmthermostat; d.fixcom = p.fixcom;
         d.fused = (p.thermostat == PIMDB_THERMO_LANGEVIN && p.nmthermostat && p.nbeads <= NM_MFMA_MAXP) ? 1 : 0;
+        if (d.fused) b->fused_any = true;
         d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
         d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
         d.ext = dv.ext; d.int_k = dv.int_k; d.strength = p.int_strength; d.cutoff = dv.cutoff;
@@ -314,8 +318,25 @@ void pimdb_batch_destroy(pimdb_batch* b) {
 
 const char* pimdb_batch_last_error(const pimdb_batch* b) { return b ? b->err.c_str() : create_error(); }
 
+}  // extern "C"
+
+// The second B and O-step that fused replicas leave owed after a step, done
+// before the host reads or replaces a state.
+static int finish_pending(pimdb_batch* b) {
+    if (!b->fused_any || !b->maybe_pending) return PIMDB_OK;
+    int rc = launch_batch(b, BM_FINISH);
+    if (rc) return rc;
+    BCK(hipStreamSynchronize(b->s));
+    b->maybe_pending = false;
+    return PIMDB_OK;
+}
+
+extern "C" {
+
 int pimdb_batch_set_state(pimdb_batch* b, int rep, const double* x, const double* p) {
     if (!b || rep < 0 || rep >= b->R) return PIMDB_E_ARG;
+    const int frc = finish_pending(b);
+    if (frc) return frc;
     const pimdb_params& q = b->prm[rep];
     const size_t n = (size_t)q.nbeads * q.ndim * q.natoms;
     std::vector<double> st(n);
@@ -338,6 +359,8 @@ int pimdb_batch_set_state(pimdb_batch* b, int rep, const double* x, const double
 
 int pimdb_batch_get_state(pimdb_batch* b, int rep, double* x, double* p, double* f) {
     if (!b || rep < 0 || rep >= b->R) return PIMDB_E_ARG;
+    const int frc = finish_pending(b);
+    if (frc) return frc;
     const pimdb_params& q = b->prm[rep];
     const size_t n = (size_t)q.nbeads * q.ndim * q.natoms;
     std::vector<double> st(n);
@@ -370,6 +393,7 @@ int pimdb_batch_step(pimdb_batch* b, int nsteps) {
     int rc = capture(b);
     if (rc) return rc;
     for (int s = 0; s < nsteps; ++s) BCK(hipGraphLaunch(b->gexec, b->s));
+    if (nsteps > 0) b->maybe_pending = true;
     return check_batch_status(b);
 }
 
@@ -380,6 +404,7 @@ int pimdb_batch_step_timed(pimdb_batch* b, int nsteps, double* device_ms) {
     BCK(hipEventRecord(b->t0, b->s));
     for (int s = 0; s < nsteps; ++s) BCK(hipGraphLaunch(b->gexec, b->s));
     BCK(hipEventRecord(b->t1, b->s));
+    b->maybe_pending = true;
     BCK(hipEventSynchronize(b->t1));
     float ms = 0.0f;
     BCK(hipEventElapsedTime(&ms, b->t0, b->t1));
@@ -404,6 +429,7 @@ int pimdb_batch_time_step(pimdb_batch* b, int reps, double* ms_step) {
     int rc = capture(b);
     if (rc) return rc;
     BCK(hipGraphLaunch(b->gexec, b->s));
+    b->maybe_pending = true;
     BCK(hipEventRecord(b->t0, b->s));
     for (int r = 0; r < reps; ++r) BCK(hipGraphLaunch(b->gexec, b->s));
     BCK(hipEventRecord(b->t1, b->s));
