@@ -256,7 +256,9 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                     }
                 }
             }
-            const int nel = P * ncols;
+            // without an external potential every force term (pairs, springs,
+            // exchange) is translation invariant: f sums to zero up to rounding
+            const int nel = r.ext.kind != POT_FREE ? P * ncols : 0;
             for (int e = threadIdx.x; e < nel; e += blockDim.x) {
                 const double fe = f[e];
                 const int ax = elem_axis<D>(e, N);

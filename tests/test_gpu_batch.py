@@ -166,3 +166,59 @@ def test_batch_rejects_mixed_dimensions():
     with pytest.raises(G.PimdbError) as ei:
         G.Batch([G.make_params(**reps[0][0]), G.make_params(**kw)])
     assert ei.value.code == G.E_ARG
+
+
+def test_batch_state_reads_between_steps():
+    """Reading the state mid-run (which completes the owed second O-step of the
+    normal-mode replicas) leaves the trajectory unchanged up to rounding; the
+    replicas include traps (forces that do not sum to zero) and fixcom."""
+    reps = nm_replicas(6)
+    bs = [G.Batch([G.make_params(**kw) for kw, _, _ in reps]) for _ in range(2)]
+    for b in bs:
+        for r, (kw, x, p) in enumerate(reps):
+            b.set_state(r, x, p)
+        b.forces()
+    bs[0].step(10)
+    bs[1].step(4)
+    for r in range(len(reps)):
+        bs[1].state(r)
+    bs[1].step(3)
+    bs[1].state(0)
+    bs[1].step(3)
+    for r in range(len(reps)):
+        xa, pa, fa = bs[0].state(r)
+        xb, pb, fb = bs[1].state(r)
+        assert rel_err(xb, xa) < 1e-12, r
+        assert rel_err(pb, pa) < 1e-11, r
+        assert per_bead_err(fb, fa) < 1e-11, r
+    for b in bs:
+        b.close()
+
+
+def test_batch_set_state_after_steps():
+    """Replacing a replica's state after steps starts it afresh: it then
+    follows the single-system form from the new state."""
+    reps = nm_replicas(3)
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    b.step(5)
+    kw, x, p = reps[1]
+    b.set_state(1, x, p)
+    b.forces()
+    c = G.Context(G.make_params(**kw))
+    c.set_state(x, p)
+    c.forces()
+    # the batch's step counter has advanced by 5: the single system skips 5 steps of noise keys
+    c.step(5)
+    c.set_state(x, p)
+    c.forces()
+    b.step(6)
+    c.step(6)
+    xb, pb, fb = b.state(1)
+    xc, pc, fc = c.state()
+    assert rel_err(xb, xc) < 1e-10
+    assert rel_err(pb, pc) < 1e-9
+    c.close()
+    b.close()
