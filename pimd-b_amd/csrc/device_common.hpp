@@ -173,7 +173,11 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
 // g(r) as above from |d| and 1/|d| with products in place of the
 // reference's divisions (norm/rm, 1/r, g/norm), the dispersion terms
 // factored as polynomials in ix^2, and exp_minimax in place of the library
-// exp: a few ulp apart.
+// exp: a few ulp apart.  For Aziz the factor epsilon/rm is folded into the
+// constants, the repulsive exponent is taken from |d| directly
+// (-alpha/rm |d|), and the range tests on r = |d|/rm are made on |d|
+// against rm-scaled bounds (at x = D the damping factor joins F = 1 with a
+// continuous first derivative).
 template <int KIND>
 __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_norm, double kpair, double strength) {
     if constexpr (KIND == POT_HARMONIC) {
@@ -182,28 +186,29 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
         const double i2 = inv_norm * inv_norm;
         return -3.0 * strength * (i2 * i2 * inv_norm);
     } else if constexpr (KIND == POT_AZIZ) {
-        constexpr double inv_rm = 1.0 / Aziz::rm;
-        const double r = norm * inv_rm;
-        const double T1 = -Aziz::A * Aziz::alpha * exp_minimax(-Aziz::alpha * r);
+        constexpr double ER = Aziz::epsilon / Aziz::rm;
+        // (epsilon/rm) T1, T1 = -A alpha exp(-alpha r)
+        const double T1 = (-Aziz::A * Aziz::alpha * ER) * exp_minimax((-Aziz::alpha / Aziz::rm) * norm);
         double g;
-        if (r > HARD_CORE_EPS && r < 0.01) {
-            g = T1 * (Aziz::epsilon / Aziz::rm);
+        if (norm > HARD_CORE_EPS * Aziz::rm && norm < 0.01 * Aziz::rm) {
+            g = T1;
         } else {
             // T2 + T3 = ix^6 [ix (6C6 + 8C8 ix^2 + 10C10 ix^4) F
             //                 - (C6 + C8 ix^2 + C10 ix^4) dF], dF = 2 D ix^2 t F
+            // (times epsilon/rm, folded into the coefficients)
             const double ix = Aziz::rm * inv_norm;
             const double ix2 = ix * ix;
             const double ix6 = ix2 * ix2 * ix2;
-            const double pa = fma_sconst(fma_sconst(10.0 * Aziz::C10, ix2, 8.0 * Aziz::C8), ix2, 6.0 * Aziz::C6);
+            const double pa =
+                fma_sconst(fma_sconst(ER * 10.0 * Aziz::C10, ix2, ER * 8.0 * Aziz::C8), ix2, ER * 6.0 * Aziz::C6);
             double attr = ix * pa;
-            if (r < Aziz::D) {
-                const double pb = fma_sconst(fma_sconst(Aziz::C10, ix2, Aziz::C8), ix2, Aziz::C6);
+            if (norm < Aziz::D * Aziz::rm) {
+                const double pb = fma_sconst(fma_sconst(ER * Aziz::C10, ix2, ER * Aziz::C8), ix2, ER * Aziz::C6);
                 const double t = Aziz::D * ix - 1.0;
                 const double ex = exp_minimax(-t * t);
                 attr = (attr - (2.0 * Aziz::D) * ix2 * t * pb) * ex;
             }
-            const double T2T3 = ix6 * attr;
-            g = (Aziz::epsilon / Aziz::rm) * (T1 + T2T3);
+            g = fma(ix6, attr, T1);
         }
         return g * inv_norm;
     } else {
