@@ -604,6 +604,29 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.cut2, r.int_k, r.strength);
     lds_double* Fw = F + (size_t)wave * bs;   // this wave's slice forces (r.plds)
+    lds_int* perm = (lds_int*)(F + (size_t)nw * bs);   // [N] tile order of the particles (r.plds)
+    if (KIND != POT_FREE && r.cutoff != 0.0 && r.plds) {
+        // Aziz with PBC: a spatial order from bead 0 (pair_order_key), which
+        // every bead shares; otherwise the identity.  The keys sit in the
+        // force arrays until the slices zero them.
+        lds_uint* key = (lds_uint*)F;
+        const bool order = KIND == POT_AZIZ && pbc;
+        const double invL = 1.0 / r.L;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            double xi[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) xi[a] = x[a * N + i];
+            key[i] = order ? pair_order_key<D>(xi, invL, N, i) : (unsigned)i;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const unsigned ki = key[i];
+            int rank = 0;
+            for (int j = 0; j < N; ++j) rank += key[j] < ki ? 1 : 0;
+            perm[rank] = i;
+        }
+        __syncthreads();
+    }
     for (int b = wave; b < P; b += nw) {
         const double* xs = x + bs * b;
         double* fo = f + bs * b;
@@ -612,9 +635,9 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
         if (pairs) {
             if (in_lds) {
                 switch (pair_image(pp)) {
-                case 0: slice_pair_forces_lds<D, KIND, 0>(xs, N, pp, lane, Fw); break;
-                case 1: slice_pair_forces_lds<D, KIND, 1>(xs, N, pp, lane, Fw); break;
-                default: slice_pair_forces_lds<D, KIND, 2>(xs, N, pp, lane, Fw); break;
+                case 0: slice_pair_forces_lds<D, KIND, 0>(xs, N, pp, lane, Fw, perm); break;
+                case 1: slice_pair_forces_lds<D, KIND, 1>(xs, N, pp, lane, Fw, perm); break;
+                default: slice_pair_forces_lds<D, KIND, 2>(xs, N, pp, lane, Fw, perm); break;
                 }
             } else {
                 switch (pair_image(pp)) {

@@ -99,6 +99,8 @@ typedef __attribute__((address_space(3))) double lds_double;
 // vmcnt only), not flat loads, whose lgkmcnt share would make the pair loop
 // wait for its own LDS force updates on every step.
 typedef const __attribute__((address_space(1))) double* gbl_cdptr;
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) unsigned lds_uint;
 
 // All pairs of one slice on one wave.  xs: slice positions [D][N]; fo: the
 // slice's force array [D][N], owned by this wave, overwritten with the pair
@@ -164,6 +166,14 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
 // it: the caller adds the rest of the force and writes the slice once).  Used
 // when every wave's slice fits the LDS.
 //
+// The tiles are cut from the particles in the order perm[0..N) (a
+// permutation of 0..N-1, in LDS): tile position k holds particle perm[k],
+// and positions and forces are addressed by the particle itself, so any
+// permutation gives the same set of pairs and the forces in particle order.
+// The caller orders the particles spatially (pair_order_keys) so that the
+// close pairs, where a branch of the pair force is taken, meet at few
+// offsets of the rotation.
+//
 // Row tiles are taken two at a time where they can be: for tile pairs (I, J)
 // and (I+1, J) with J >= I+2, lane l holds rows 64I+l and 64(I+1)+l and one
 // rotating column of tile J, so the column's DPP rotation and its LDS update
@@ -172,9 +182,10 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
 template <int D, int KIND, int IMG>
 __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
                                                bool diag, const double (&xi)[D], bool row_ok, double (&racc)[D],
-                                               lds_double* F) {
+                                               lds_double* F, const lds_int* perm) {
     const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
-    const int jc0 = J * PTILE + ((lane + s0) & (PTILE - 1));
+    const int kc = J * PTILE + ((lane + s0) & (PTILE - 1));
+    const int jc0 = kc < N ? perm[kc] : N;
     // the column's index, as its first element in F (jd = jc D), moves with it
     unsigned jd = (unsigned)(jc0 * D);
     double xj[D];
@@ -202,8 +213,9 @@ __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const 
 template <int D, int KIND, int IMG>
 __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
                                                    const double (&xa)[D], const double (&xb)[D], double (&ra)[D],
-                                                   double (&rb)[D], lds_double* F) {
-    const int jc0 = J * PTILE + lane;
+                                                   double (&rb)[D], lds_double* F, const lds_int* perm) {
+    const int kc = J * PTILE + lane;
+    const int jc0 = kc < N ? perm[kc] : N;
     unsigned jd = (unsigned)(jc0 * D);
     double xj[D];
 #pragma unroll
@@ -234,15 +246,16 @@ __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, co
 
 template <int D, int KIND, int IMG>
 __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const PairParams& pp, int lane,
-                                                      lds_double* F) {
+                                                      lds_double* F, const lds_int* perm) {
     const gbl_cdptr xs = (gbl_cdptr)xs_;
     for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
     __builtin_amdgcn_wave_barrier();
     const int T = (N + PTILE - 1) / PTILE;
     for (int I = 0; I < T; I += 2) {
         const bool two = I + 1 < T;
-        const int ia = I * PTILE + lane, ib = (I + 1) * PTILE + lane;
-        const bool oka = ia < N, okb = two && ib < N;
+        const int ka = I * PTILE + lane, kb = (I + 1) * PTILE + lane;
+        const bool oka = ka < N, okb = two && kb < N;
+        const int ia = oka ? perm[ka] : N, ib = okb ? perm[kb] : N;
         double xa[D], xb[D], ra[D], rb[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -251,12 +264,13 @@ __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, 
             ra[a] = 0.0;
             rb[a] = 0.0;
         }
-        pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I, true, xa, oka, ra, F);
+        pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I, true, xa, oka, ra, F, perm);
         if (two) {
-            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, false, xa, oka, ra, F);
-            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, true, xb, okb, rb, F);
+            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, false, xa, oka, ra, F, perm);
+            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, true, xb, okb, rb, F, perm);
             // a tile J >= I+2 exists only when tiles I and I+1 are full
-            for (int J = I + 2; J < T; ++J) pair_tile_two_rows<D, KIND, IMG>(xs, N, pp, lane, J, xa, xb, ra, rb, F);
+            for (int J = I + 2; J < T; ++J)
+                pair_tile_two_rows<D, KIND, IMG>(xs, N, pp, lane, J, xa, xb, ra, rb, F, perm);
         }
         if (oka) {
 #pragma unroll
@@ -270,6 +284,31 @@ __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, 
         }
     }
     __builtin_amdgcn_wave_barrier();
+}
+
+// Spatial order for the tiles of slice_pair_forces_lds: the key of particle
+// i (position xi, box L) is its cell in a row-major grid of about one cell
+// per particle, 4 cells along the first axis, and the particle index below
+// it, so the keys are distinct.  Four cells along the first axis put a
+// particle's neighbours at tile offsets +-1 and +-3 (that axis, periodic),
+// +-4 (the second) and +-4 ny (the third): close pairs then meet at few
+// rotation offsets, and tiles two or more apart along the last axis hold no
+// close pairs at all.
+template <int D>
+__device__ __forceinline__ unsigned pair_order_key(const double (&xi)[D], double invL, int N, int i) {
+    int ny = 1, nz = 1;
+    if constexpr (D == 2) ny = max(1, (N + 3) / 4);
+    if constexpr (D == 3) ny = nz = max(1, (int)sqrt((double)N / 4.0 + 0.5));
+    const int n[3] = {D == 1 ? N : 4, ny, nz};
+    unsigned cell = 0;
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+        double f = xi[a] * invL;
+        f -= floor(f);
+        const int c = min(n[a] - 1, (int)(f * n[a]));
+        cell = cell * (unsigned)n[a] + (unsigned)c;
+    }
+    return (cell << 16) | (unsigned)i;
 }
 
 }  // namespace pimdb

@@ -242,7 +242,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         }
         need = std::max(need, batch_nm_lds_doubles(p.nbeads, 1024 / WAVE));
         // pair forces of every wave's slice in LDS when they fit
-        const size_t pl = (size_t)(1024 / WAVE) * p.ndim * p.natoms;
+        const size_t pl = (size_t)(1024 / WAVE) * p.ndim * p.natoms + (p.natoms + 1) / 2;   // + the tile order
         b->desc[r].plds = sizeof(double) * (16 + pl) <= device_lds_limit() ? 1 : 0;
         if (b->desc[r].plds) need = std::max(need, pl);
     }
