@@ -189,27 +189,29 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
         constexpr double ER = Aziz::epsilon / Aziz::rm;
         // (epsilon/rm) T1, T1 = -A alpha exp(-alpha r)
         const double T1 = (-Aziz::A * Aziz::alpha * ER) * exp_minimax((-Aziz::alpha / Aziz::rm) * norm);
-        double g;
-        if (norm > HARD_CORE_EPS * Aziz::rm && norm < 0.01 * Aziz::rm) {
-            g = T1;
-        } else {
-            // T2 + T3 = ix^6 [ix (6C6 + 8C8 ix^2 + 10C10 ix^4) F
-            //                 - (C6 + C8 ix^2 + C10 ix^4) dF], dF = 2 D ix^2 t F
-            // (times epsilon/rm, folded into the coefficients)
-            const double ix = Aziz::rm * inv_norm;
-            const double ix2 = ix * ix;
-            const double ix6 = ix2 * ix2 * ix2;
-            const double pa =
-                fma_sconst(fma_sconst(ER * 10.0 * Aziz::C10, ix2, ER * 8.0 * Aziz::C8), ix2, ER * 6.0 * Aziz::C6);
-            double attr = ix * pa;
-            if (norm < Aziz::D * Aziz::rm) {
+        // T2 + T3 = ix^6 [ix (6C6 + 8C8 ix^2 + 10C10 ix^4) F
+        //                 - (C6 + C8 ix^2 + C10 ix^4) dF], dF = 2 D ix^2 t F
+        // (times epsilon/rm, folded into the coefficients)
+        const double ix = Aziz::rm * inv_norm;
+        const double ix2 = ix * ix;
+        const double ix6 = ix2 * ix2 * ix2;
+        const double pa =
+            fma_sconst(fma_sconst(ER * 10.0 * Aziz::C10, ix2, ER * 8.0 * Aziz::C8), ix2, ER * 6.0 * Aziz::C6);
+        double attr = ix * pa;
+        if (norm < Aziz::D * Aziz::rm) {
+            // the reference's hard-core range (r in (eps, 0.01), inside the
+            // damping range) keeps the repulsive term alone: attr = 0 gives
+            // g = T1 exactly (ix6 is finite there)
+            if (norm > HARD_CORE_EPS * Aziz::rm && norm < 0.01 * Aziz::rm) {
+                attr = 0.0;
+            } else {
                 const double pb = fma_sconst(fma_sconst(ER * Aziz::C10, ix2, ER * Aziz::C8), ix2, ER * Aziz::C6);
                 const double t = Aziz::D * ix - 1.0;
                 const double ex = exp_minimax(-t * t);
                 attr = (attr - (2.0 * Aziz::D) * ix2 * t * pb) * ex;
             }
-            g = fma(ix6, attr, T1);
         }
+        const double g = fma(ix6, attr, T1);
         return g * inv_norm;
     } else {
         return 0.0;

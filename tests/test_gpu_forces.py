@@ -73,3 +73,34 @@ def test_forces_translation_invariance_without_trap():
     f = gpu_forces(kw, x)
     tot = f.sum(axis=(0, 1))
     assert np.max(np.abs(tot)) < 1e-10 * np.max(np.abs(f)) * f.shape[0] * f.shape[1]
+
+
+@pytest.mark.parametrize("bosonic", [0, 1])
+def test_forces_aziz_ranges(bosonic):
+    """Aziz pairs placed in each range of the force: the hard-core range
+    r/rm in (1e-7, 0.01) (repulsive term only, aziz.cpp), the damped range
+    r/rm < D = 1.241314 on both sides of 0.01 and just below D, just above D,
+    and the undamped tail; each pair far from the others."""
+    rm = 5.60738
+    seps = np.array([0.005, 0.0099, 0.0101, 0.3, 0.9, 1.2413, 1.2414, 1.6, 2.5]) * rm
+    n = len(seps)
+    P = 3
+    rng = np.random.default_rng(17)
+    x = np.zeros((P, 2 * n, 3))
+    for k, s in enumerate(seps):
+        c = np.array([60.0 * k, 0.0, 0.0])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for b in range(P):
+            x[b, 2 * k] = c
+            x[b, 2 * k + 1] = c + s * u
+    kw = dict(natoms=2 * n, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0, gamma=1e-3,
+              bosonic=bosonic, pbc=0, size=100.0, ext_kind="free", int_kind="aziz", cutoff=-1.0,
+              propagator="cartesian", thermostat="langevin", seed=3)
+    f = gpu_forces(kw, x)
+    fo = O.update_forces(O.make_params(**kw), x)
+    # pair by pair: each pair's force relative to its own scale
+    for k in range(n):
+        sl = slice(2 * k, 2 * k + 2)
+        err = np.max(np.abs(f[:, sl] - fo[:, sl])) / np.max(np.abs(fo[:, sl]))
+        assert err < 1e-10, (seps[k] / rm, err)
