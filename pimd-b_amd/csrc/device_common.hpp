@@ -128,6 +128,32 @@ __device__ __forceinline__ double exp_minimax(double x) {
     return ldexp(p, (int)k);
 }
 
+// e^x for x <= 0 to within 5e-14 relative, for the pair force's
+// exponentials (whose results enter forces compared at 1e-10 relative):
+// Cody-Waite reduction with ln 2 in one double (its 2.3e-17 relative error,
+// times |k| <= 1076, bounds the reduction's share by 2.5e-14; 1.7e-15 for
+// x >= -51, the Aziz repulsion's range inside L/2) and a degree-9 minimax
+// polynomial on |r| <= ln(2)/2 (pimd-b_amd/csrc/gen_exp_minimax.py 9: 1.36e-14
+// relative with the coefficients rounded to double).  Three instructions
+// fewer than exp_minimax.  Arguments below -746 give 0.
+__device__ __forceinline__ double exp_pair(double x) {
+    const double LOG2E = 1.4426950408889634;
+    const double LN2 = 6.931471805599453e-01;
+    x = fmax(x, -746.0);
+    const double k = rint(x * LOG2E);
+    const double r = fma(-k, LN2, x);
+    double p = fma_sconst(2.7482187570680464e-06, r, 2.488401002638418e-05);
+    p = fma_sconst(p, r, 1.9841596395525616e-04);
+    p = fma_sconst(p, r, 1.3888802828792633e-03);
+    p = fma_sconst(p, r, 8.333333002823904e-03);
+    p = fma_sconst(p, r, 4.166666703246359e-02);
+    p = fma_sconst(p, r, 1.6666666667826951e-01);
+    p = fma_sconst(p, r, 4.999999999945797e-01);
+    p = fma_sconst(p, r, 9.999999999998928e-01);
+    p = fma_sconst(p, r, 1.0000000000000127);
+    return ldexp(p, (int)k);
+}
+
 // Scalar factor g(r) of the interaction gradient, gradV(d) = g * d
 // (harmonic.cpp:21-23, dipole.cpp:24-45, aziz.cpp:56-106).  `norm` is |d|.
 template <int KIND>
@@ -172,8 +198,8 @@ __device__ __forceinline__ double pair_grad_factor(double norm, double kpair, do
 
 // g(r) as above from |d| and 1/|d| with products in place of the
 // reference's divisions (norm/rm, 1/r, g/norm), the dispersion terms
-// factored as polynomials in ix^2, and exp_minimax in place of the library
-// exp: a few ulp apart.  For Aziz the factor epsilon/rm is folded into the
+// factored as polynomials in ix^2, and exp_pair in place of the library exp:
+// within a few 1e-14 relative.  For Aziz the factor epsilon/rm is folded into the
 // constants, the repulsive exponent is taken from |d| directly
 // (-alpha/rm |d|), and the range tests on r = |d|/rm are made on |d|
 // against rm-scaled bounds (at x = D the damping factor joins F = 1 with a
@@ -188,7 +214,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
     } else if constexpr (KIND == POT_AZIZ) {
         constexpr double ER = Aziz::epsilon / Aziz::rm;
         // (epsilon/rm) T1, T1 = -A alpha exp(-alpha r)
-        const double T1 = (-Aziz::A * Aziz::alpha * ER) * exp_minimax((-Aziz::alpha / Aziz::rm) * norm);
+        const double T1 = (-Aziz::A * Aziz::alpha * ER) * exp_pair((-Aziz::alpha / Aziz::rm) * norm);
         // T2 + T3 = ix^6 [ix (6C6 + 8C8 ix^2 + 10C10 ix^4) F
         //                 - (C6 + C8 ix^2 + C10 ix^4) dF], dF = 2 D ix^2 t F
         // (times epsilon/rm, folded into the coefficients)
@@ -207,7 +233,7 @@ __device__ __forceinline__ double pair_grad_factor_fast(double norm, double inv_
             } else {
                 const double pb = fma_sconst(fma_sconst(ER * Aziz::C10, ix2, ER * Aziz::C8), ix2, ER * Aziz::C6);
                 const double t = Aziz::D * ix - 1.0;
-                const double ex = exp_minimax(-t * t);
+                const double ex = exp_pair(-t * t);
                 attr = (attr - (2.0 * Aziz::D) * ix2 * t * pb) * ex;
             }
         }

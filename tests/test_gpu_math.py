@@ -45,6 +45,22 @@ def test_exp_subnormal_and_underflow():
     assert np.all(y[400:] == 0.0)
 
 
+def test_exp_pair_relative_bound():
+    """The Aziz force's e^x (x <= 0): within 5e-14 relative of the correctly
+    rounded value (normal results), 2e-14 on the repulsion's range x >= -51;
+    0 below -746."""
+    rng = np.random.default_rng(13)
+    x = np.concatenate([rng.uniform(-51.0, 0.0, 1500), rng.uniform(-708.0, -51.0, 800),
+                        -np.arange(0, 60) * np.log(2.0) / 2, [0.0, -1e-300, -5e-324, -0.5 * np.log(2.0)]])
+    y = G.math_probe("exp_pair", x)
+    ref = correctly_rounded(mp.exp, x)
+    rel = np.abs(y - ref) / ref
+    near = x >= -51.0
+    assert rel[near].max() <= 2e-14, (x[near][np.argmax(rel[near])], rel[near].max())
+    assert rel.max() <= 5e-14, (x[np.argmax(rel)], rel.max())
+    assert np.all(G.math_probe("exp_pair", np.array([-746.5, -750.0, -1e4, -1e300, -np.inf])) == 0.0)
+
+
 def test_rsqrt_within_one_ulp():
     rng = np.random.default_rng(12)
     x = np.concatenate([10.0 ** rng.uniform(-300, 300, 1500), rng.uniform(1.0, 500.0, 1500),
