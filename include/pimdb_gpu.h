@@ -161,7 +161,8 @@ int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gb
  * fn 0 = e^x (exp_minimax), fn 1 = 1/sqrt(x) (rsqrt_refined), fn 2 = ln x
  * for x in (0, 1] (log_unit), fn 3 / 4 = sin / cos of 2 pi x for x in [0, 1)
  * (sincos_2pi; these three make the Box-Muller noise), fn 5 = e^x for
- * x <= 0 (exp_pair, the Aziz force's exponentials), y = f(x).  A test
+ * x <= 0 (exp_pair, the Aziz force's exponentials), fn 6 = 1/sqrt(x)
+ * (rsqrt_pair, the pair forces' 1/r), y = f(x).  A test
  * hook for their accuracy.  No reference counterpart (the reference calls the
  * C library's exp and sqrt, aziz.cpp:56-106, and draws its noise on the host,
  * random_mars.cpp:83-103). */

@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(256) k_probe_math(int fn, const double* __rest
     case 1: v = pimdb::rsqrt_refined(x[i]); break;
     case 2: v = pimdb::log_unit(x[i]); break;
     case 5: v = pimdb::exp_pair(x[i]); break;
+    case 6: v = pimdb::rsqrt_pair(x[i]); break;
     default: {
         double sn, cs;
         pimdb::sincos_2pi(x[i], sn, cs);
@@ -126,7 +127,7 @@ extern "C" int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, doub
 }
 
 extern "C" int pimdb_math_probe(int fn, const double* x, double* y, long long n) {
-    if (fn < 0 || fn > 5 || n < 0 || (n > 0 && (!x || !y))) return PIMDB_E_ARG;
+    if (fn < 0 || fn > 6 || n < 0 || (n > 0 && (!x || !y))) return PIMDB_E_ARG;
     if (n == 0) return PIMDB_OK;
     double *dx = nullptr, *dy = nullptr;
     int rc = PIMDB_OK;

@@ -306,10 +306,11 @@ def math_probe(fn, x):
     """The device elementary functions evaluated on the GPU at the values x:
     the pair kernels' e^x (fn "exp") and 1/sqrt(x) ("rsqrt"), and the noise's
     ln x on (0, 1] ("log_unit") and sin / cos of 2 pi x on [0, 1)
-    ("sin_2pi", "cos_2pi"), and the Aziz force's e^x for x <= 0 ("exp_pair")."""
+    ("sin_2pi", "cos_2pi"), and the pair forces' e^x for x <= 0 ("exp_pair")
+    and 1/sqrt(x) ("rsqrt_pair")."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
-    fns = {"exp": 0, "rsqrt": 1, "log_unit": 2, "sin_2pi": 3, "cos_2pi": 4, "exp_pair": 5}
+    fns = {"exp": 0, "rsqrt": 1, "log_unit": 2, "sin_2pi": 3, "cos_2pi": 4, "exp_pair": 5, "rsqrt_pair": 6}
     rc = lib().pimdb_math_probe(fns[fn], _p(x), _p(y), int(x.size))
     if rc != OK:
         _raise(rc, "pimdb_math_probe failed")

@@ -61,6 +61,17 @@ def test_exp_pair_relative_bound():
     assert np.all(G.math_probe("exp_pair", np.array([-746.5, -750.0, -1e4, -1e300, -np.inf])) == 0.0)
 
 
+def test_rsqrt_pair_relative_bound():
+    """The pair forces' 1/sqrt(x): within 2e-14 relative."""
+    rng = np.random.default_rng(14)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 1500), rng.uniform(1.0, 2000.0, 1500),
+                        [1.0, 2.0, 4.0, 0.5, 1e-300, 1e300]])
+    y = G.math_probe("rsqrt_pair", x)
+    ref = correctly_rounded(lambda v: 1 / mp.sqrt(v), x)
+    rel = np.abs(y - ref) / ref
+    assert rel.max() <= 2e-14, (x[np.argmax(rel)], rel.max())
+
+
 def test_rsqrt_within_one_ulp():
     rng = np.random.default_rng(12)
     x = np.concatenate([10.0 ** rng.uniform(-300, 300, 1500), rng.uniform(1.0, 500.0, 1500),
