@@ -157,8 +157,9 @@ int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_e
  * be NULL.  No reference counterpart. */
 int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gbs, long long copy_bytes);
 
-/* Evaluates the pair kernels' device elementary functions on n host values:
- * fn 0 = e^x (exp_minimax), fn 1 = 1/sqrt(x) (rsqrt_refined), fn 2 = ln x
+/* Evaluates the device elementary functions on n host values:
+ * fn 0 = e^x (exp_minimax, the exchange's and integrator's e^x), fn 1 =
+ * 1/sqrt(x) to 1 ulp (rsqrt_refined, the full-precision form of fn 6), fn 2 = ln x
  * for x in (0, 1] (log_unit), fn 3 / 4 = sin / cos of 2 pi x for x in [0, 1)
  * (sincos_2pi; these three make the Box-Muller noise), fn 5 = e^x for
  * x <= 0 (exp_pair, the Aziz force's exponentials), fn 6 = 1/sqrt(x)

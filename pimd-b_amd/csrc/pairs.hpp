@@ -61,8 +61,8 @@ __device__ __forceinline__ double min_image_pair(double dx, const PairParams& pp
 
 __host__ __device__ inline int pair_image(const PairParams& pp) { return !pp.pbc ? 0 : (pp.guard ? 2 : 1); }
 
-// 1/sqrt(r2) to double precision: the hardware estimate y0 and one
-// third-order correction.  With e = 1 - r2 y0^2, 1/sqrt(r2) = y0 (1-e)^(-1/2)
+// 1/sqrt(r2) to double precision (to 1 ulp; the pair forces use the cheaper
+// rsqrt_pair below): the hardware estimate y0 and one third-order correction.  With e = 1 - r2 y0^2, 1/sqrt(r2) = y0 (1-e)^(-1/2)
 // = y0 (1 + e/2 + 3e^2/8 + 5e^3/16 + ...); the dropped term is 5e^3/16, far
 // below an ulp for the estimate's |e| ~ 2^-23.
 __device__ __forceinline__ double rsqrt_refined(double r2) {
@@ -179,7 +179,7 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
 // permutation of 0..N-1, in LDS): tile position k holds particle perm[k],
 // and positions and forces are addressed by the particle itself, so any
 // permutation gives the same set of pairs and the forces in particle order.
-// The caller orders the particles spatially (pair_order_keys) so that the
+// The caller orders the particles spatially (pair_order_key) so that the
 // close pairs, where a branch of the pair force is taken, meet at few
 // offsets of the rotation.
 //

@@ -304,7 +304,8 @@ def peak_probe(copy_bytes=1 << 30):
 
 def math_probe(fn, x):
     """The device elementary functions evaluated on the GPU at the values x:
-    the pair kernels' e^x (fn "exp") and 1/sqrt(x) ("rsqrt"), and the noise's
+    e^x to 1 ulp (fn "exp", the exchange's and integrator's), 1/sqrt(x) to 1
+    ulp ("rsqrt", the full-precision form of "rsqrt_pair"), the noise's
     ln x on (0, 1] ("log_unit") and sin / cos of 2 pi x on [0, 1)
     ("sin_2pi", "cos_2pi"), and the pair forces' e^x for x <= 0 ("exp_pair")
     and 1/sqrt(x) ("rsqrt_pair")."""
