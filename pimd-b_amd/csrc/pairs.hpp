@@ -200,8 +200,9 @@ __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const 
     double xj[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
-    for (int s = s0; s < s1; ++s) {
-        if (row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32)) {
+    // the pair of this lane at the current offset, if `act`
+    auto step = [&](bool act) {
+        if (act) {
             double d[D], g;
             if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
                 lds_double* Fj = F + jd;
@@ -215,6 +216,15 @@ __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const 
 #pragma unroll
         for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
         jd = (unsigned)lane_rotate((int)jd);
+    };
+    if ((J + 1) * PTILE <= N) {
+        // full row and column tiles (J >= the row tile): every lane's pair
+        // exists, except at a diagonal tile's offset 32 (lanes < 32 only)
+        const int s2 = diag ? 32 : PTILE;
+        for (int s = s0; s < s2; ++s) step(true);
+        if (diag) step(lane < 32);
+    } else {
+        for (int s = s0; s < s1; ++s) step(row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32));
     }
 }
 
