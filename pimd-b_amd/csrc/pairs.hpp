@@ -239,8 +239,8 @@ __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, co
     double xj[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
-    for (int s = 0; s < PTILE; ++s) {
-        if (jd < (unsigned)(N * D)) {
+    auto step = [&](bool act) {
+        if (act) {
             double da[D], db[D], ga = 0.0, gb = 0.0;
             const bool ina = pair_grad<D, KIND, IMG>(xa, xj, pp, da, ga);
             const bool inb = pair_grad<D, KIND, IMG>(xb, xj, pp, db, gb);
@@ -260,6 +260,11 @@ __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, co
 #pragma unroll
         for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
         jd = (unsigned)lane_rotate((int)jd);
+    };
+    if ((J + 1) * PTILE <= N) {   // a full column tile: every lane's column exists
+        for (int s = 0; s < PTILE; ++s) step(true);
+    } else {
+        for (int s = 0; s < PTILE; ++s) step(jd < (unsigned)(N * D));
     }
 }
 
