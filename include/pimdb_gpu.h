@@ -146,7 +146,12 @@ int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinc
 int pimdb_graph_capture(pimdb_ctx* ctx);
 
 /* Average device time (ms) of `reps` launches of the physical-force kernel
- * of one force evaluation, measured with HIP events on the context stream.
+ * of one force evaluation (ms_forces), of the exchange forward pass and
+ * exterior forces (ms_exchange) and of one MD step with device noise
+ * (ms_step), measured with HIP events on the context stream; any output may
+ * be NULL.  Side effects: the forces are recomputed from the current
+ * positions after the force and exchange timings, so those leave the state
+ * as it was; ms_step advances the state by 2 * reps device-noise steps.
  * No reference counterpart (the reference times only the run loop,
  * src/simulation.cpp:225-226, 282-289). */
 int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_exchange, double* ms_step);
@@ -156,6 +161,12 @@ int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_e
  * (GB/s, read + write, over copy_bytes; <= 0 means 1 GiB).  Any output may
  * be NULL.  No reference counterpart. */
 int pimdb_peak_probe(double* fma_tflops, double* exp_gexp_per_s, double* copy_gbs, long long copy_bytes);
+
+/* Compute units of the current device and the LDS bytes one workgroup may
+ * use (the batched form plans its replicas' phases for that LDS, shared by
+ * the workgroups of one CU).  Any output may be NULL.  No reference
+ * counterpart. */
+int pimdb_device_info(int* compute_units, long long* lds_bytes);
 
 /* Evaluates the device elementary functions on n host values:
  * fn 0 = e^x (exp_minimax, the exchange's and integrator's e^x), fn 1 =
@@ -209,10 +220,13 @@ int pimdb_batch_forces(pimdb_batch* batch);
 int pimdb_batch_step(pimdb_batch* batch, int nsteps);
 /* Workgroup size of the persistent kernel (256, 512 or 1024; 0 keeps the
  * current one) and number of workgroups (0 keeps the current one; default
- * one per compute unit). */
+ * one per compute unit).  More workgroups than compute units share each CU's
+ * LDS, and every replica's phases are planned for that share; PIMDB_E_ARG
+ * (the previous shape kept) when some replica does not fit it. */
 int pimdb_batch_configure(pimdb_batch* batch, int threads, int resident);
-/* Times trial_steps steps for each workgroup size and 1 or 2 workgroups per
- * compute unit, keeps the fastest and restores the state it started from. */
+/* Times trial_steps steps for each launch shape (256, 512 or 1024 threads,
+ * 1, 2 or 4 workgroups per compute unit, at most 1024 threads per CU) whose
+ * LDS plan fits, keeps the fastest and restores the state it started from. */
 int pimdb_batch_autotune(pimdb_batch* batch, int trial_steps, int* threads_out, int* resident_out);
 /* nsteps MD steps of every replica, as pimdb_batch_step, with the device
  * time between events recorded on the batch's stream just before the first

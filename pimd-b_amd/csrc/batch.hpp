@@ -25,6 +25,7 @@ struct RepDesc {
     int xdual;                  // exchange recursions side by side (LDS permitting)
     int plds;                   // pair forces accumulated in LDS (LDS permitting)
     int fused;                  // normal-mode O-step on the matrix cores with B + A fused in (see wg_momenta)
+    int xslots;                 // exterior-force chunk slots that the workgroup's LDS holds (>= 1)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
     double int_k, strength, cutoff, cut2;
@@ -86,7 +87,7 @@ __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
     u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.xdual = uni(r.xdual);
-    u.plds = uni(r.plds); u.fused = uni(r.fused);
+    u.plds = uni(r.plds); u.fused = uni(r.fused); u.xslots = uni(r.xslots);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
     u.int_k = uni(r.int_k); u.strength = uni(r.strength); u.cutoff = uni(r.cutoff); u.cut2 = uni(r.cut2);
@@ -689,9 +690,11 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
 // on entry.
 template <int D>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
-                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_, int dual_) {
+                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_, int dual_,
+                                         int slots_) {
     const int N = uni(N_), P = uni(P_);
     const bool dual = uni(dual_) != 0;
+    const int slots = uni(slots_);
     const bool pbc = uni(pbc_) != 0;
     const double L = uni(L_), k = uni(k_), bt = uni(bt_);
     const double* x = uni(x_);
@@ -708,21 +711,21 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     if (dual) {
         // the two recursions side by side: even waves forward, odd waves
         // backward (their in-block chains then sit on different SIMDs)
-        const RecDual rd = rec_carve_dual(lds, N, D);
-        const Ring r = ring_stage<D>(rd.s0, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
         const int wave = uni((int)(threadIdx.x / WAVE)), nw = blockDim.x / WAVE;
+        const RecDual rd = rec_carve_dual(lds, N, D, nw / 2);
+        const Ring r = ring_stage<D>(rd.s0, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
         const int gtid = (wave >> 1) * WAVE + (int)(threadIdx.x % WAVE), gnthr = (nw / 2) * WAVE;
         if ((wave & 1) == 0) exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s0, gtid, gnthr);
         else exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s1, gtid, gnthr);
         __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D));
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D), slots);
     } else {
-        const RecShared sh = rec_carve(lds, N, D);
+        const RecShared sh = rec_carve(lds, N, D, blockDim.x / WAVE);
         const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
         exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
         exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
         __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D));
+        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D), slots);
     }
 }
 
@@ -752,7 +755,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.Vb + r.off_V, bp.status + ri, lds, r.xdual);
+                           bp.Vb + r.off_V, bp.status + ri, lds, r.xdual, r.xslots);
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];

@@ -188,7 +188,8 @@ inline ForceArgs make_force_args(const pimdb_params& p, const Derived& d) {
     return fa;
 }
 
-inline size_t rec_lds_bytes(int N, int D) { return sizeof(double) * rec_shared_doubles(N, D); }
+// LDS of k_exchange_recursions (1024 threads per direction).
+inline size_t rec_lds_bytes(int N, int D) { return sizeof(double) * rec_shared_doubles(N, D, 1024 / WAVE); }
 
 inline size_t device_lds_limit() {
     static size_t lim = 0;

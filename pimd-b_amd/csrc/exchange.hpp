@@ -218,8 +218,8 @@ struct RecShared {
     double* U;         // [N+1]
     double* lg;        // [N+1]  ln(i)
     double* R;         // [REC_B] shifts of the block's targets
-    double* part;      // [REC_MAXWAVES][REC_B] per-wave partial min of the earlier sources
-    double* psum;      // [REC_MAXWAVES][REC_B] per-wave partial sums scaled to that min, then to R
+    double* part;      // [nw][REC_B] per-wave partial min of the earlier sources
+    double* psum;      // [nw][REC_B] per-wave partial sums scaled to that min, then to R
     double* a;         // [REC_B*REC_LD] in-block weights, then coefficients, a[j*REC_LD + t]
     double* x1;        // [D][N] bead 1
     double* xP;        // [D][N] bead P
@@ -228,48 +228,52 @@ struct RecShared {
 };
 
 // Doubles of the staged ring (beads 1 and P, split prefix sums) and of one
-// direction's recursion workspace.
+// direction's recursion workspace for a group of nw waves.
 __host__ __device__ inline size_t rec_ring_doubles(int N, int D) { return (size_t)(2 * D + 2) * N; }
-__host__ __device__ inline size_t rec_dir_doubles(int N) {
-    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)REC_MAXWAVES * REC_B + (size_t)REC_B * REC_LD;
+__host__ __device__ inline size_t rec_dir_doubles(int N, int nw) {
+    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)nw * REC_B + (size_t)REC_B * REC_LD;
 }
-// One workgroup running one recursion at a time: the ring, then one workspace.
-__host__ __device__ inline size_t rec_shared_doubles(int N, int D) { return rec_ring_doubles(N, D) + rec_dir_doubles(N); }
+// One workgroup of nw waves running one recursion at a time: the ring, then
+// one workspace.
+__host__ __device__ inline size_t rec_shared_doubles(int N, int D, int nw) {
+    return rec_ring_doubles(N, D) + rec_dir_doubles(N, nw);
+}
 
-__device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N) {
+__device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N, int nw) {
     s.U = p; p += N + 1;
     s.lg = p; p += N + 1;
     s.R = p; p += REC_B;
-    s.part = p; p += (size_t)REC_MAXWAVES * REC_B;
-    s.psum = p; p += (size_t)REC_MAXWAVES * REC_B;
+    s.part = p; p += (size_t)nw * REC_B;
+    s.psum = p; p += (size_t)nw * REC_B;
     s.a = p; p += (size_t)REC_B * REC_LD;
 }
 
 // Ring first, so that after the recursions everything from
 // base + rec_ring_doubles(N, D) on is free (the exterior forces' scratch).
-__device__ __forceinline__ RecShared rec_carve(double* base, int N, int D) {
+// nw: the waves of the group that will run the recursion.
+__device__ __forceinline__ RecShared rec_carve(double* base, int N, int D, int nw) {
     RecShared s;
     double* p = base;
     s.x1 = p; p += (size_t)D * N;
     s.xP = p; p += (size_t)D * N;
     s.Shi = p; p += N;
     s.Slo = p; p += N;
-    rec_carve_dir(s, p, N);
+    rec_carve_dir(s, p, N, nw);
     return s;
 }
 
 // Workspace of the two recursions run side by side by two halves of one
-// workgroup (the batched form): the staged ring, then each direction's own
-// U, ln i, R, partials and in-block matrix.
+// workgroup (the batched form), nw waves each: the staged ring, then each
+// direction's own U, ln i, R, partials and in-block matrix.
 struct RecDual {
     RecShared s0, s1;
 };
-__device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D) {
+__device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D, int nw) {
     RecDual rd;
-    rd.s0 = rec_carve(base, N, D);
-    double* p = base + rec_shared_doubles(N, D);
+    rd.s0 = rec_carve(base, N, D, nw);
+    double* p = base + rec_shared_doubles(N, D, nw);
     rd.s1 = rd.s0;
-    rec_carve_dir(rd.s1, p, N);
+    rec_carve_dir(rd.s1, p, N, nw);
     return rd;
 }
 
@@ -405,7 +409,8 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
     const int tid = gtid, nthr = gnthr;
     // the wave index in a scalar register: loops over it are uniform
     const int lane = tid % WAVE, wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
-    const int nwave = min(nthr / WAVE, REC_MAXWAVES);
+    // the partials are sized for the group's waves (rec_carve)
+    const int nwave = nthr / WAVE;
     const int ntarget = DIR == 0 ? N : N - 1;
     for (int i = tid; i <= N; i += nthr) {
         sh.lg[i] = (i > 0) ? log((double)i) : 0.0;
@@ -538,7 +543,7 @@ static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, doub
                                                                      double* __restrict__ Shi, double* __restrict__ Slo,
                                                                      int* __restrict__ status) {
     extern __shared__ double smem[];
-    const RecShared sh = rec_carve(smem, N, D);
+    const RecShared sh = rec_carve(smem, N, D, blockDim.x / WAVE);
     const bool fwd = blockIdx.x == 0;
     const Ring r = ring_stage<D>(sh, N, x1, xP, k, pbc != 0, L, fwd ? Shi : nullptr, fwd ? Slo : nullptr);
     if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
@@ -647,7 +652,7 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
 // are added per particle in chunk order, so the result depends on the data
 // only.  The links b = a - 1 (probability 1 - exp(...)) and the interior links
 // are added per particle at the end: f[bead][a][l] += k (S + links).
-// Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x).
+// Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x, slot_cap).
 constexpr int EXT_SLOTS = 32;
 
 // Chunk length: the longest of 64, 32, 16, 8 offsets that gives every wave a
@@ -658,18 +663,27 @@ __host__ __device__ inline int ext_chunk(int N, int nw) {
     while (ch > 8 && ntile * (WAVE / ch) < nw) ch >>= 1;
     return ch;
 }
-__host__ __device__ inline int ext_slots(int N, int nw) {
+// Chunks per round: every chunk if they fit, at most EXT_SLOTS, and at most
+// `cap` (the slots the caller's LDS holds, >= 1).
+__host__ __device__ inline int ext_slots(int N, int nw, int cap) {
     const int T = (N + WAVE - 1) / WAVE, ntile = T * (T + 1) / 2;
     const int nchunk = ntile * (WAVE / ext_chunk(N, nw));
-    return nchunk < EXT_SLOTS ? nchunk : EXT_SLOTS;
+    int s = nchunk < EXT_SLOTS ? nchunk : EXT_SLOTS;
+    s = s < cap ? s : cap;
+    return s > 1 ? s : 1;
 }
 // Column record of particle b: xP[b] (D), H_b, L_b, -beta~ (Vb[b+1] - V[N]),
 // 1/(b+1); an odd stride, so that lanes reading consecutive records spread
 // over the LDS banks.
 constexpr int EXT_REC = 9;
-__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads) {
-    const int nw = nthreads / WAVE;
-    return 2 * (size_t)N + 1 + (size_t)EXT_REC * N + 2 * (size_t)D * N + (size_t)ext_slots(N, nw) * 2 * D * WAVE;
+// Scratch of the exterior forces: a part that grows with N, plus one slot of
+// chunk partials per chunk of a round.
+__host__ __device__ inline size_t ext_fixed_doubles(int N, int D) {
+    return 2 * (size_t)N + 1 + (size_t)EXT_REC * N + 2 * (size_t)D * N;
+}
+__host__ __device__ inline size_t ext_slot_doubles(int D) { return 2 * (size_t)D * WAVE; }
+__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads, int cap) {
+    return ext_fixed_doubles(N, D) + (size_t)ext_slots(N, nthreads / WAVE, cap) * ext_slot_doubles(D);
 }
 
 // Tile (A, B), A <= B, of index `tile` in row-major order over T x T, and back.
@@ -687,7 +701,8 @@ template <int D>
 __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, const Ring& r,
                                                    const double* __restrict__ V, const double* __restrict__ Vb,
                                                    const double* __restrict__ x2, const double* __restrict__ xPm1,
-                                                   double* __restrict__ f1, double* __restrict__ fP, double* scratch) {
+                                                   double* __restrict__ f1, double* __restrict__ fP, double* scratch,
+                                                   int slot_cap) {
     const int nthr = blockDim.x, tid = threadIdx.x;
     const int lane = tid % WAVE, wave = __builtin_amdgcn_readfirstlane(tid / WAVE), nw = nthr / WAVE;
     static_assert(D + 4 <= EXT_REC, "column record");
@@ -717,7 +732,7 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
     const int T = (N + WAVE - 1) / WAVE;
     const int ch = ext_chunk(N, nw), per_tile = WAVE / ch;
     const int nchunk = T * (T + 1) / 2 * per_tile;
-    const int slots = ext_slots(N, nw);
+    const int slots = ext_slots(N, nw, slot_cap);
     for (int c0 = 0; c0 < nchunk; c0 += slots) {
         const int cend = min(nchunk, c0 + slots);
         for (int c = c0 + wave; c < cend; c += nw) {   // wave-uniform

@@ -23,6 +23,7 @@ struct pimdb_batch {
     int R = 0, D = 3, kind = 0;
     int threads = 1024, resident = 0;   // resident workgroups (0: all CUs)
     int cus = 0;
+    int per_cu = 1;                     // workgroups that share one CU's LDS
     size_t nx = 0, nV = 0, nC = 0;
     size_t lds = 0;
     BatchPtrs bp{};
@@ -78,8 +79,78 @@ static void build_schedule(pimdb_batch* b, std::vector<int>& sched, std::vector<
     }
 }
 
+// LDS plan of every replica for `threads` threads per workgroup and
+// `per_cu` workgroups per compute unit (each then gets 1/per_cu of its LDS):
+// which phases keep their work in LDS (the exchange recursions side by side,
+// the pair forces of every wave's slice), how many chunk slots the exterior
+// forces get, and the largest need.  Returns the dynamic LDS bytes, or 0
+// (with *why) when some replica does not fit.
+static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepDesc>& desc, std::string* why) {
+    const size_t budget = device_lds_limit() / (size_t)per_cu / sizeof(double);   // doubles
+    const size_t avail = budget > 16 ? budget - 16 : 0;                             // after the reduction slots
+    const int nw = threads / WAVE;
+    size_t need = 0;
+    for (int r = 0; r < b->R; ++r) {
+        const pimdb_params& p = b->prm[r];
+        const int N = p.natoms, D = p.ndim;
+        RepDesc& d = desc[r];
+        d.xdual = 0;
+        d.xslots = 1;
+        if (d.bosonic) {
+            // recursion workspace, and after it (over the same LDS) the
+            // exterior forces' scratch with as many chunk slots as fit
+            const size_t ring = rec_ring_doubles(N, D);
+            const size_t fix = ext_fixed_doubles(N, D), slot = ext_slot_doubles(D);
+            const int smax = ext_slots(N, nw, EXT_SLOTS);
+            auto fits = [&](size_t rec, int min_slots, int& slots) {
+                if (ring + rec > avail || ring + fix + (size_t)min_slots * slot > avail) return false;
+                const size_t room = (avail - ring - fix) / slot;
+                slots = (int)std::min<size_t>((size_t)smax, room);
+                return true;
+            };
+            int slots = 0;
+            const size_t dual = 2 * rec_dir_doubles(N, nw / 2), single = rec_dir_doubles(N, nw);
+            if (nw >= 2 && fits(dual, std::min(smax, 4), slots)) {
+                d.xdual = 1;
+                need = std::max(need, ring + std::max(dual, fix + (size_t)slots * slot));
+            } else if (fits(single, 1, slots)) {
+                need = std::max(need, ring + std::max(single, fix + (size_t)slots * slot));
+            } else {
+                *why = "replica " + std::to_string(r) + ": natoms too large for the exchange workspace of a "
+                       "batched workgroup";
+                return 0;
+            }
+            d.xslots = slots;
+        }
+        const size_t nm = batch_nm_lds_doubles(p.nbeads, nw);
+        if (nm > avail) {
+            *why = "replica " + std::to_string(r) + ": nbeads too large for the batched workgroup";
+            return 0;
+        }
+        need = std::max(need, nm);
+        // pair forces of every wave's slice in LDS when they fit (+ the tile order)
+        const size_t pl = (size_t)nw * D * N + (N + 1) / 2;
+        d.plds = pl <= avail ? 1 : 0;
+        if (d.plds) need = std::max(need, pl);
+    }
+    return sizeof(double) * (16 + need);
+}
+
 static int upload_schedule(pimdb_batch* b) {
-    b->grid = std::max(1, std::min(b->R, b->resident > 0 ? b->resident : b->cus));
+    const int resident = b->resident > 0 ? b->resident : b->cus;
+    b->grid = std::max(1, std::min(b->R, resident));
+    // the LDS is planned for the residency asked for (each CU's share)
+    b->per_cu = (resident + b->cus - 1) / b->cus;
+    {
+        std::vector<RepDesc> desc = b->desc;
+        std::string why;
+        const size_t lds = plan_lds(b, b->threads, b->per_cu, desc, &why);
+        if (!lds) return b->fail(PIMDB_E_ARG, why);
+        b->desc = desc;
+        b->lds = lds;
+        if (b->d_desc)
+            BCK(hipMemcpy(b->d_desc, b->desc.data(), sizeof(RepDesc) * b->R, hipMemcpyHostToDevice));
+    }
     std::vector<int> sched, off;
     build_schedule(b, sched, off);
     if (b->d_sched) (void)hipFree(b->d_sched);
@@ -228,27 +299,15 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         b->desc.push_back(d);
         Nmax = std::max(Nmax, p.natoms);
     }
-    size_t need = 0;
-    for (int r = 0; r < nrep; ++r) {
-        const pimdb_params& p = reps[r];
-        if (b->desc[r].bosonic) {
-            // both recursions side by side when their workspace fits the LDS
-            // (the exterior forces' scratch overlays the recursion workspace)
-            const size_t ring = rec_ring_doubles(p.natoms, p.ndim);
-            const size_t ext = ext_scratch_doubles(p.natoms, p.ndim, 1024);
-            const size_t dual = ring + std::max(2 * rec_dir_doubles(p.natoms), ext);
-            b->desc[r].xdual = sizeof(double) * (16 + dual) <= device_lds_limit() ? 1 : 0;
-            need = std::max(need, b->desc[r].xdual ? dual : ring + std::max(rec_dir_doubles(p.natoms), ext));
-        }
-        need = std::max(need, batch_nm_lds_doubles(p.nbeads, 1024 / WAVE));
-        // pair forces of every wave's slice in LDS when they fit
-        const size_t pl = (size_t)(1024 / WAVE) * p.ndim * p.natoms + (p.natoms + 1) / 2;   // + the tile order
-        b->desc[r].plds = sizeof(double) * (16 + pl) <= device_lds_limit() ? 1 : 0;
-        if (b->desc[r].plds) need = std::max(need, pl);
-    }
-    b->lds = sizeof(double) * (16 + need);
     int rc = PIMDB_OK;
-    if (b->lds > device_lds_limit()) rc = b->fail(PIMDB_E_ARG, "natoms too large for the batched workgroup");
+    {
+        // the default launch shape (1024 threads, one workgroup per CU) must fit
+        std::vector<RepDesc> desc = b->desc;
+        std::string why;
+        b->lds = plan_lds(b, b->threads, 1, desc, &why);
+        if (!b->lds) rc = b->fail(PIMDB_E_ARG, why);
+        else b->desc = desc;
+    }
     auto alloc = [&](double** ptr, size_t n) {
         if (rc) return;
         if (hipMalloc((void**)ptr, std::max<size_t>(1, n) * sizeof(double)) != hipSuccess ||
@@ -414,14 +473,23 @@ int pimdb_batch_step_timed(pimdb_batch* b, int nsteps, double* device_ms) {
 
 int pimdb_batch_configure(pimdb_batch* b, int threads, int resident) {
     if (!b) return PIMDB_E_ARG;
-    if (threads) {
-        if (threads != 256 && threads != 512 && threads != 1024) return b->fail(PIMDB_E_ARG, "threads must be 256, 512 or 1024");
-        b->threads = threads;
-    }
+    if (threads && threads != 256 && threads != 512 && threads != 1024)
+        return b->fail(PIMDB_E_ARG, "threads must be 256, 512 or 1024");
     if (resident < 0) return b->fail(PIMDB_E_ARG, "resident must be >= 0");
-    if (resident) b->resident = resident;
     BCK(hipStreamSynchronize(b->s));
-    return upload_schedule(b);
+    const int old_threads = b->threads, old_resident = b->resident;
+    if (threads) b->threads = threads;
+    if (resident) b->resident = resident;
+    const int rc = upload_schedule(b);
+    if (rc == PIMDB_E_ARG) {   // this shape's LDS plan does not fit: keep the previous one
+        const std::string why = b->err;
+        b->threads = old_threads;
+        b->resident = old_resident;
+        const int rc2 = upload_schedule(b);
+        if (rc2) return rc2;
+        return b->fail(PIMDB_E_ARG, why);
+    }
+    return rc;
 }
 
 int pimdb_batch_time_step(pimdb_batch* b, int reps, double* ms_step) {
@@ -498,22 +566,34 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
     std::vector<double> scmp(8 * (size_t)b->R);
     BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
-    const int cand_threads[3] = {256, 512, 1024};
+    // launch shapes: threads per workgroup x workgroups per CU, at most 16
+    // waves per CU (the kernel's 128 VGPRs allow 4 waves per SIMD); shapes
+    // whose LDS plan does not fit, or that give the same grid as one already
+    // timed, are skipped
+    const int cand[6][2] = {{1024, 1}, {512, 1}, {512, 2}, {256, 1}, {256, 2}, {256, 4}};
     double best = 1e300;
     int bt = b->threads, br = b->resident;
     int rc = PIMDB_OK;
-    for (int ti = 0; ti < 3 && !rc; ++ti) {
-        for (int per_cu = 1; per_cu <= 2 && !rc; ++per_cu) {
-            b->threads = cand_threads[ti];
-            b->resident = b->cus * per_cu;
-            rc = upload_schedule(b);
-            double ms = 0.0;
-            if (!rc) rc = pimdb_batch_time_step(b, trial_steps, &ms);
-            if (!rc && ms < best) {
-                best = ms;
-                bt = b->threads;
-                br = b->resident;
-            }
+    std::vector<std::pair<int, int>> tried;
+    for (int ci = 0; ci < 6 && !rc; ++ci) {
+        const int threads = cand[ci][0], per_cu = cand[ci][1], resident = b->cus * per_cu;
+        const int grid = std::max(1, std::min(b->R, resident));
+        // more workgroups per CU than the batch fills only tightens the plan
+        if (per_cu > 1 && grid <= b->cus * (per_cu - 1)) continue;
+        if (std::find(tried.begin(), tried.end(), std::make_pair(threads, grid)) != tried.end()) continue;
+        tried.push_back({threads, grid});
+        std::vector<RepDesc> desc = b->desc;
+        std::string why;
+        if (!plan_lds(b, threads, per_cu, desc, &why)) continue;
+        b->threads = threads;
+        b->resident = resident;
+        rc = upload_schedule(b);
+        double ms = 0.0;
+        if (!rc) rc = pimdb_batch_time_step(b, trial_steps, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            bt = b->threads;
+            br = b->resident;
         }
     }
     b->threads = bt;

@@ -719,6 +719,12 @@ int pimdb_time_kernels(pimdb_ctx* ctx, int reps, double* ms_forces, double* ms_e
             if (rc) return rc;
         }
     }
+    if (ms_forces || ms_exchange) {
+        // the force timing overwrote f with the physical and spring part
+        // alone: the full forces of the current positions again
+        rc = enqueue_forces(ctx);
+        if (rc) return rc;
+    }
     if (ms_step) {
         // whole steps (device noise), as the benchmark replays them; this
         // advances the state by 2 * reps steps

@@ -145,3 +145,14 @@ extern "C" int pimdb_math_probe(int fn, const double* x, double* y, long long n)
     if (dy) (void)hipFree(dy);
     return rc;
 }
+
+extern "C" int pimdb_device_info(int* compute_units, long long* lds_bytes) {
+    int dev = 0, ndev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1 || hipGetDevice(&dev) != hipSuccess ||
+        hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return PIMDB_E_HIP;
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (lds_bytes) *lds_bytes = (long long)prop.sharedMemPerBlock;
+    return PIMDB_OK;
+}

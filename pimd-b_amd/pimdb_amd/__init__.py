@@ -96,6 +96,7 @@ def lib():
         "pimdb_time_kernels": (i, [vp, i, _DP, _DP, _DP]),
         "pimdb_peak_probe": (i, [_DP, _DP, _DP, C.c_longlong]),
         "pimdb_math_probe": (i, [i, _DP, _DP, C.c_longlong]),
+        "pimdb_device_info": (i, [C.POINTER(i), C.POINTER(C.c_longlong)]),
         "pimdb_xplan_create": (i, [i, i, d, d, i, d, C.POINTER(vp)]),
         "pimdb_xplan_destroy": (None, [vp]),
         "pimdb_xplan_forward": (i, [vp, vp, vp, vp, vp, vp, vp]),
@@ -300,6 +301,19 @@ def peak_probe(copy_bytes=1 << 30):
     if rc != OK:
         _raise(rc, "pimdb_peak_probe failed")
     return tuple(float(a[0]) for a in v)
+
+
+def device_info():
+    """(compute units, LDS bytes per workgroup) of the current device."""
+    cu, lds = C.c_int(), C.c_longlong()
+    rc = lib().pimdb_device_info(C.byref(cu), C.byref(lds))
+    if rc != OK:
+        _raise(rc, "pimdb_device_info failed")
+    return cu.value, lds.value
+
+
+def device_cus():
+    return device_info()[0]
 
 
 def math_probe(fn, x):
