@@ -24,6 +24,15 @@
  *       - RanMars vs the reference's own random_mars.cpp compiled in oracle/_ref
  *       - NM matrices orthogonality; free-ring exact rotation
  *   observables - simulation.out rows of the fixtures (rtol 1e-5)
+ *   topology probabilities (orc_distinct_probability, orc_longest_probability)
+ *       - their closed meaning over the compositions of the ring: the
+ *         all-singleton share w(singletons)/Z and N w(one block)/Z, by
+ *         enumeration for N <= 10 (tests/test_oracle_pins.py); N=1, N=2,
+ *         coincident beads (1/N!, 1) and far-apart particles (1, 0)
+ *   pair terms of the observables (int_pot, the r_i-only pair virial,
+ *       energy.cpp:56-111) - worked by hand for 2-3 particles with harmonic,
+ *       dipole and Aziz pairs, PBC and a pair past the cutoff
+ *       (tests/test_oracle_pins.py)
  *   zeroMomentum - sums beads sequentially; the reference reduces them with
  *       MPI_Allreduce whose order is MPI-implementation defined.  All fixtures
  *       run with fixcom=false except the one initial call; that order is

@@ -37,6 +37,26 @@ def test_observables_aziz(replicate):
     compare(kw, x, p)
 
 
+@pytest.mark.parametrize("kind", ["aziz", "dipole", "harmonic"])
+def test_observables_pairs_in_a_trap(kind):
+    """int_pot and the pair virial reported separately (both potentials
+    non-free): Aziz under PBC with a cutoff, dipole and harmonic pairs without."""
+    if kind == "aziz":
+        kw, x = aziz_config(64, 8, 1.5, replicate=False, seed=41)
+        kw.update(ext_kind="harmonic", ext_omega=3.0e-3 * 0.036749326)
+    else:
+        rng = np.random.default_rng(42)
+        N, P = 24, 8
+        grid = np.array([(a, b, c) for a in range(3) for b in range(3) for c in range(3)][:N], float) * 4.0
+        x = np.stack([grid - grid.mean(0) + rng.normal(0, 0.3, size=grid.shape) for _ in range(P)])
+        kw = dict(natoms=N, nbeads=P, ndim=3, mass=7296.3, temperature=2.0 * KELVIN, dt=20.0, gamma=5e-4,
+                  bosonic=1, pbc=0, size=40.0, ext_kind="harmonic", ext_omega=1.1e-4, int_kind=kind,
+                  int_omega=3.7e-5, int_strength=1.3, cutoff=-1.0, propagator="cartesian",
+                  thermostat="langevin", seed=9)
+    p = np.random.default_rng(2).normal(0, 0.3, size=x.shape)
+    compare(kw, x, p)
+
+
 @pytest.mark.parametrize("ext", ["double_well", "cosine"])
 def test_observables_external_traps(ext):
     """Energy, virial and their ext/int split with the reference's double-well
