@@ -195,6 +195,21 @@ int pimdb_xplan_forward(pimdb_xplan* plan, const double* x1, const double* xP, d
                         double* Vb, void* stream);
 int pimdb_xplan_vb_grad(pimdb_xplan* plan, const double* x1, const double* xP, double* grad, void* stream);
 
+/* pimdb_xplan_create + pimdb_xplan_forward (VB only) + pimdb_xplan_vb_grad in
+ * one call that allocates nothing: the same kernels, with their workspace
+ * (V, Vb, the ring's link prefix sums, the overflow flag) in a fixed device
+ * array of the library (64 KB per GPU, held until the process ends).  x1, xP,
+ * VB (may be NULL) and grad (may be NULL: forward only) are device arrays as
+ * above.  natoms <= 2048 (and within the recursion's LDS workspace), else
+ * PIMDB_E_ARG; the message of any failure is pimdb_last_error(NULL).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other threads) race.  Synchronises `stream` before returning.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc). */
+int pimdb_exchange_vb_grad_noalloc(int natoms, int ndim, double beta_t, double spring_k, int pbc, double size,
+                                   const double* x1, const double* xP, double* VB, double* grad, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batched replicas: many independent systems of uneven N, P and T advanced */
 /* by one persistent kernel per MD step (device noise), replayed from a     */

@@ -101,6 +101,7 @@ def lib():
         "pimdb_xplan_destroy": (None, [vp]),
         "pimdb_xplan_forward": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "pimdb_xplan_vb_grad": (i, [vp, vp, vp, vp, vp]),
+        "pimdb_exchange_vb_grad_noalloc": (i, [i, i, d, d, i, d, vp, vp, vp, vp, vp]),
         "pimdb_batch_create": (i, [C.POINTER(Params), i, C.POINTER(vp)]),
         "pimdb_batch_destroy": (None, [vp]),
         "pimdb_batch_last_error": (C.c_char_p, [vp]),
@@ -360,3 +361,14 @@ class XPlan:
         rc = lib().pimdb_xplan_vb_grad(self.h, x1_ptr, xP_ptr, grad_ptr, stream)
         if rc != OK:
             _raise(rc, "pimdb_xplan_vb_grad failed")
+
+
+def exchange_vb_grad_noalloc(natoms, ndim, beta_t, spring_k, pbc, size, x1_ptr, xP_ptr, VB_ptr=None,
+                             grad_ptr=None, stream=None):
+    """V_B and -dV_B/dx of beads 1 and P on device arrays (SoA [axis][particle];
+    grad [2][D][N]) with no device allocation: the library's fixed per-GPU
+    workspace, shared by every call on a GPU (overlapping calls race)."""
+    rc = lib().pimdb_exchange_vb_grad_noalloc(int(natoms), int(ndim), float(beta_t), float(spring_k), int(pbc),
+                                              float(size), x1_ptr, xP_ptr, VB_ptr, grad_ptr, stream)
+    if rc != OK:
+        _raise(rc, lib().pimdb_last_error(None).decode())

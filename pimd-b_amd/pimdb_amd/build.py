@@ -6,7 +6,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpimdb_gpu.so")
-SOURCES = ["pimdb_ctx.hip", "pimdb_xplan.hip", "pimdb_probe.hip", "pimdb_batch.hip"]
+SOURCES = ["pimdb_ctx.hip", "pimdb_xplan.hip", "pimdb_probe.hip", "pimdb_batch.hip", "pimdb_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
