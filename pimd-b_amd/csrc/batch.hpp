@@ -604,6 +604,10 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const PairParams pp = make_pair_params(r.L, r.pbc, r.cutoff, r.cut2, r.int_k, r.strength);
+    // the fractional-coordinate constants, wave-uniform (scalar registers)
+    FracParams fp = make_frac_params(pp);
+    fp.cut2 = uni(fp.cut2); fp.kh = uni(fp.kh); fp.kd = uni(fp.kd); fp.ex = uni(fp.ex);
+    fp.b0 = uni(fp.b0); fp.b1 = uni(fp.b1); fp.b2 = uni(fp.b2); fp.dl = uni(fp.dl);
     lds_double* Fw = F + (size_t)wave * bs;   // this wave's slice forces (r.plds)
     lds_int* perm = (lds_int*)(F + (size_t)nw * bs);   // [N] tile order of the particles (r.plds)
     if (KIND != POT_FREE && r.cutoff != 0.0 && r.plds) {
@@ -636,9 +640,9 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
         if (pairs) {
             if (in_lds) {
                 switch (pair_image(pp)) {
-                case 0: slice_pair_forces_lds<D, KIND, 0>(xs, N, pp, lane, Fw, perm); break;
-                case 1: slice_pair_forces_lds<D, KIND, 1>(xs, N, pp, lane, Fw, perm); break;
-                default: slice_pair_forces_lds<D, KIND, 2>(xs, N, pp, lane, Fw, perm); break;
+                case 0: slice_pair_forces_lds<D, KIND, 0>(xs, N, WalkGrad<D, KIND, 0>{pp, fp}, lane, Fw, perm); break;
+                case 1: slice_pair_forces_lds<D, KIND, 1>(xs, N, WalkGrad<D, KIND, 1>{pp, fp}, lane, Fw, perm); break;
+                default: slice_pair_forces_lds<D, KIND, 2>(xs, N, WalkGrad<D, KIND, 2>{pp, fp}, lane, Fw, perm); break;
                 }
             } else {
                 switch (pair_image(pp)) {

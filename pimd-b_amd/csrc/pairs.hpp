@@ -101,6 +101,123 @@ __device__ __forceinline__ bool pair_grad(const double (&xi)[D], const double (&
     return true;
 }
 
+// ---- the pair gradient in box-fractional coordinates (PBC with a cutoff) ----
+// With positions s = x / L the minimum image is t - rint(t), t = s_i - s_j
+// (a subtraction and a rounding per axis instead of four operations), and
+// every power of L folds into per-box constants: gradV(d) = g d = g_s d_s
+// with d_s = d / L and g_s = g L.  With y = L / r (one reciprocal square root
+// of |d_s|^2), q = y^2 and r_s = r / L:
+//   harmonic  g_s = k_pair L
+//   dipole    g_s = (-3 strength / L^4) q^2 y
+//   Aziz      g_s = y T1 + U                       (r >= D rm)
+//             g_s = y T1 + (U - t y q^4 pc(q)) F   (r < D rm; U alone outside
+//                                                   the reference's hard-core
+//                                                   range r/rm in (1e-7, 0.01))
+// where, with ER = epsilon / rm and a = rm / L (pair_grad_factor_fast's
+// expression rewritten in y):
+//   T1 = -A alpha ER exp(-alpha r / rm)              (the repulsion)
+//   U  = q^4 (b0 + b1 q + b2 q^2),  b = ER a^7 (6 C6, 8 C8 a^2, 10 C10 a^4)
+//   pc = 2 D ER a^8 (C6 + C8 a^2 q + C10 a^4 q^2)
+//      = D a (b0 / 3 + b1 q / 4 + b2 q^2 / 5)
+//   t  = D a y - 1,  F = exp(-t^2)                   (aziz.h:25-34)
+// The damping-range terms are formed from b and D a inside that branch, so
+// the loop holds six per-box constants.
+// Both exponentials are 2^(k + f), k = rint(a), f = a - k exactly, with a the
+// exponent in base 2 and 2^f from frac_exp2 below.  The cutoff test compares
+// |d_s|^2 with (cutoff threshold) / L^2.
+struct FracParams {
+    double cut2;            // squared cutoff threshold over L^2
+    double kh, kd;          // harmonic and dipole factors
+    double ex;              // -alpha L / rm * log2(e): the repulsion's exponent per unit r_s
+    double b0, b1, b2;      // undamped dispersion
+    double dl;              // D rm / L: the damping range in r_s, and t = dl y - 1
+};
+
+__host__ __device__ inline FracParams make_frac_params(const PairParams& pp) {
+    FracParams f;
+    const double L = pp.L, iL = pp.invL;
+    f.cut2 = pp.cut2 * iL * iL;
+    f.kh = pp.int_k * L;
+    f.kd = -3.0 * pp.strength * (iL * iL) * (iL * iL);
+    const double ER = Aziz::epsilon / Aziz::rm;
+    const double a = Aziz::rm * iL, a2 = a * a, a7 = a * a2 * a2 * a2;
+    f.ex = -Aziz::alpha * (L / Aziz::rm) * 1.4426950408889634;
+    f.b0 = 6.0 * Aziz::C6 * ER * a7;
+    f.b1 = 8.0 * Aziz::C8 * ER * a7 * a2;
+    f.b2 = 10.0 * Aziz::C10 * ER * a7 * a2 * a2;
+    f.dl = Aziz::D * a;
+    return f;
+}
+
+// The Aziz repulsion's prefactor -A alpha epsilon / rm, folded into frac_exp2.
+constexpr double AZ_T1C = -Aziz::A * Aziz::alpha * (Aziz::epsilon / Aziz::rm);
+constexpr double LN2D = 6.931471805599453e-01;
+
+// AZ_T1C 2^f for |f| <= 1/2: exp_pair's degree-9 minimax polynomial for e^r
+// on |r| <= ln(2)/2 taken at r = f ln 2 (coefficient j times (ln 2)^j) and
+// scaled by AZ_T1C, so the same relative accuracy (1.4e-14, plus a few ulp
+// from the folded constants).
+__device__ __forceinline__ double frac_exp2(double f) {
+    constexpr double l1 = LN2D, l2 = l1 * l1, l3 = l2 * l1, l4 = l3 * l1, l5 = l4 * l1, l6 = l5 * l1,
+                     l7 = l6 * l1, l8 = l7 * l1, l9 = l8 * l1;
+    double p = fma_sconst(AZ_T1C * 2.7482187570680464e-06 * l9, f, AZ_T1C * 2.488401002638418e-05 * l8);
+    p = fma_sconst(p, f, AZ_T1C * 1.9841596395525616e-04 * l7);
+    p = fma_sconst(p, f, AZ_T1C * 1.3888802828792633e-03 * l6);
+    p = fma_sconst(p, f, AZ_T1C * 8.333333002823904e-03 * l5);
+    p = fma_sconst(p, f, AZ_T1C * 4.166666703246359e-02 * l4);
+    p = fma_sconst(p, f, AZ_T1C * 1.6666666667826951e-01 * l3);
+    p = fma_sconst(p, f, AZ_T1C * 4.999999999945797e-01 * l2);
+    p = fma_sconst(p, f, AZ_T1C * 9.999999999998928e-01 * l1);
+    p = fma_sconst(p, f, AZ_T1C * 1.0000000000000127);
+    return p;
+}
+
+// gradV = g d for the pair (i, j) in fractional coordinates (xi, xj = x / L):
+// d = min-imaged s_i - s_j and g = g_s as above when the pair is inside the
+// cutoff (the return value), g = 0 otherwise.  Branch-free but for the
+// damping range: outside the cutoff r > L/2 and every term is finite, and in
+// the hard-core range the damped terms are finite with F = 0.
+template <int D, int KIND>
+__device__ __forceinline__ bool pair_grad_frac(const double (&xi)[D], const double (&xj)[D], const FracParams& fp,
+                                               double (&d)[D], double& g) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const double t = xi[a] - xj[a];
+        d[a] = t - rint(t);
+    }
+    const double s = norm2_fma<D>(d);
+    const bool in = s < fp.cut2;
+    if constexpr (KIND == POT_HARMONIC) {
+        g = fp.kh;
+    } else if constexpr (KIND == POT_DIPOLE) {
+        const double y = rsqrt_pair(s), q = y * y;
+        g = fp.kd * (q * q * y);
+    } else if constexpr (KIND == POT_AZIZ) {
+        const double y = rsqrt_pair(s);
+        const double rs = s * y;
+        const double e1 = fp.ex * rs, k1 = rint(e1);
+        const double T1 = ldexp(frac_exp2(e1 - k1), (int)k1);
+        const double q = y * y, q2 = q * q, q4 = q2 * q2;
+        double att = q4 * fma(fma(fp.b2, q, fp.b1), q, fp.b0);
+        if (rs < fp.dl) {
+            const double t = fma(fp.dl, y, -1.0);
+            const double e2 = (t * -1.4426950408889634) * t, k2 = rint(e2);
+            const double F = ldexp(frac_exp2(e2 - k2), (int)k2) * (1.0 / AZ_T1C);
+            const double pc = fma(fma(fp.b2 * 0.2, q, fp.b1 * 0.25), q, fp.b0 * (1.0 / 3.0)) * fp.dl;
+            const double damped = fma(-(t * y) * q4, pc, att) * F;
+            // r / rm = rs / a and a = dl / D: the reference's hard-core range
+            // (1e-7, 0.01) keeps the repulsion alone
+            constexpr double HLO = HARD_CORE_EPS / Aziz::D, HHI = 0.01 / Aziz::D;
+            att = (rs > HLO * fp.dl && rs < HHI * fp.dl) ? 0.0 : damped;
+        }
+        g = fma(y, T1, att);
+    } else {
+        g = 0.0;
+    }
+    g = in ? g : 0.0;
+    return in;
+}
+
 // LDS pointer type: accesses through it are ds_ instructions that the
 // compiler knows cannot alias global memory.
 typedef __attribute__((address_space(3))) double lds_double;
@@ -170,6 +287,25 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
     }
 }
 
+// The pair gradient of the LDS pair walk: pair_grad on the positions as
+// stored or, with periodic boundaries and a cutoff (IMG 1), pair_grad_frac on
+// the positions scaled by 1/L as they are loaded.  Both give gradV = g d with
+// g d in force units.
+template <int D, int KIND, int IMG>
+struct WalkGrad {
+    PairParams pp;
+    FracParams fp;
+    __device__ __forceinline__ double load(const gbl_cdptr xs, int idx) const {
+        if constexpr (IMG == 1) return xs[idx] * pp.invL;
+        else return xs[idx];
+    }
+    __device__ __forceinline__ bool operator()(const double (&xi)[D], const double (&xj)[D], double (&d)[D],
+                                               double& g) const {
+        if constexpr (IMG == 1) return pair_grad_frac<D, KIND>(xi, xj, fp, d, g);
+        else return pair_grad<D, KIND, IMG>(xi, xj, pp, d, g);
+    }
+};
+
 // All pairs of one slice on one wave, with the column forces accumulated in
 // the wave's own LDS array F[N][D] (zeroed here; the pair forces are left in
 // it: the caller adds the rest of the force and writes the slice once).  Used
@@ -189,9 +325,9 @@ __device__ __forceinline__ void slice_pair_forces(const double* xs, double* fo, 
 // (the two pairs' terms added first) serve two pairs.  The tiles (I, I),
 // (I, I+1), (I+1, I+1) and an odd last row tile take one row at a time.
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
-                                               bool diag, const double (&xi)[D], bool row_ok, double (&racc)[D],
-                                               lds_double* F, const lds_int* perm) {
+__device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const WalkGrad<D, KIND, IMG>& wg, int lane,
+                                               int J, bool diag, const double (&xi)[D], bool row_ok,
+                                               double (&racc)[D], lds_double* F, const lds_int* perm) {
     const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
     const int kc = J * PTILE + ((lane + s0) & (PTILE - 1));
     const int jc0 = kc < N ? perm[kc] : N;
@@ -199,12 +335,12 @@ __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const 
     unsigned jd = (unsigned)(jc0 * D);
     double xj[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
+    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? wg.load(xs, a * N + jc0) : 0.0;
     // the pair of this lane at the current offset, if `act`
     auto step = [&](bool act) {
         if (act) {
             double d[D], g;
-            if (pair_grad<D, KIND, IMG>(xi, xj, pp, d, g)) {
+            if (wg(xi, xj, d, g)) {
                 lds_double* Fj = F + jd;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -230,20 +366,21 @@ __device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const 
 
 // Rows 64I+l and 64(I+1)+l (both present) against the full column tile J.
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, const PairParams& pp, int lane, int J,
-                                                   const double (&xa)[D], const double (&xb)[D], double (&ra)[D],
-                                                   double (&rb)[D], lds_double* F, const lds_int* perm) {
+__device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, const WalkGrad<D, KIND, IMG>& wg,
+                                                   int lane, int J, const double (&xa)[D], const double (&xb)[D],
+                                                   double (&ra)[D], double (&rb)[D], lds_double* F,
+                                                   const lds_int* perm) {
     const int kc = J * PTILE + lane;
     const int jc0 = kc < N ? perm[kc] : N;
     unsigned jd = (unsigned)(jc0 * D);
     double xj[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? xs[a * N + jc0] : 0.0;
+    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? wg.load(xs, a * N + jc0) : 0.0;
     auto step = [&](bool act) {
         if (act) {
             double da[D], db[D], ga = 0.0, gb = 0.0;
-            const bool ina = pair_grad<D, KIND, IMG>(xa, xj, pp, da, ga);
-            const bool inb = pair_grad<D, KIND, IMG>(xb, xj, pp, db, gb);
+            const bool ina = wg(xa, xj, da, ga);
+            const bool inb = wg(xb, xj, db, gb);
             if (ina || inb) {
                 if (!ina) ga = 0.0;
                 if (!inb) gb = 0.0;
@@ -269,8 +406,8 @@ __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, co
 }
 
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const PairParams& pp, int lane,
-                                                      lds_double* F, const lds_int* perm) {
+__device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const WalkGrad<D, KIND, IMG>& wg,
+                                                      int lane, lds_double* F, const lds_int* perm) {
     const gbl_cdptr xs = (gbl_cdptr)xs_;
     for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
     __builtin_amdgcn_wave_barrier();
@@ -283,18 +420,18 @@ __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, 
         double xa[D], xb[D], ra[D], rb[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            xa[a] = oka ? xs[a * N + ia] : 0.0;
-            xb[a] = okb ? xs[a * N + ib] : 0.0;
+            xa[a] = oka ? wg.load(xs, a * N + ia) : 0.0;
+            xb[a] = okb ? wg.load(xs, a * N + ib) : 0.0;
             ra[a] = 0.0;
             rb[a] = 0.0;
         }
-        pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I, true, xa, oka, ra, F, perm);
+        pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I, true, xa, oka, ra, F, perm);
         if (two) {
-            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, false, xa, oka, ra, F, perm);
-            pair_tile_rows<D, KIND, IMG>(xs, N, pp, lane, I + 1, true, xb, okb, rb, F, perm);
+            pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I + 1, false, xa, oka, ra, F, perm);
+            pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I + 1, true, xb, okb, rb, F, perm);
             // a tile J >= I+2 exists only when tiles I and I+1 are full
             for (int J = I + 2; J < T; ++J)
-                pair_tile_two_rows<D, KIND, IMG>(xs, N, pp, lane, J, xa, xb, ra, rb, F, perm);
+                pair_tile_two_rows<D, KIND, IMG>(xs, N, wg, lane, J, xa, xb, ra, rb, F, perm);
         }
         if (oka) {
 #pragma unroll
