@@ -78,6 +78,7 @@ def _load():
         "orc_exchange": (i, [i, i, d, d, i, d, _DP, _DP, _DP, _DP,
                              _DP, _DP, _DP, _DP, _DP, _DP]),
         "orc_vb_enumerate": (d, [i, d, _DP]),
+        "orc_exchange_quad": (i, [i, i, d, d, i, d, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP]),
         "orc_factorial_e_shift": (d, [i, i, d, i, d, _DP, _DP]),
         "orc_factorial_vb": (d, [i, i, d, d, i, d, _DP, _DP]),
         "orc_factorial_force_last": (v, [i, i, d, d, i, d, _DP, _DP, _DP, d, _DP]),
@@ -160,6 +161,20 @@ def exchange(x1, x2, xPm1, xP, beta_t, k, pbc=False, L=0.0):
                             *(ptr(out[n]) for n in ("E", "V", "Vb", "prob", "f_first", "f_last")))
     if rc != 0:
         raise OracleError("exchange returned %d" % rc)
+    return out
+
+
+def exchange_quad(x1, x2, xPm1, xP, beta_t, k, pbc=False, L=0.0):
+    """orc_exchange in quadruple precision (the exact values for the double
+    inputs): dict(V, Vb, prob, f_first, f_last)."""
+    x1, x2, xPm1, xP = (f64(a) for a in (x1, x2, xPm1, xP))
+    N, D = x1.shape
+    out = dict(V=np.zeros(N + 1), Vb=np.zeros(N + 1), prob=np.zeros((N, N)), f_first=np.zeros((N, D)),
+               f_last=np.zeros((N, D)))
+    rc = lib().orc_exchange_quad(N, D, beta_t, k, int(pbc), L, ptr(x1), ptr(x2), ptr(xPm1), ptr(xP),
+                                 *(ptr(out[n]) for n in ("V", "Vb", "prob", "f_first", "f_last")))
+    if rc != 0:
+        raise OracleError("exchange_quad returned %d" % rc)
     return out
 
 

@@ -96,6 +96,13 @@ int orc_exchange(int N, int D, double beta_t, double k, int pbc, double L,
                  const double* x1, const double* x2, const double* xPm1, const double* xP,
                  double* E_kn, double* V, double* Vb, double* prob,
                  double* f_first, double* f_last);
+
+/* orc_exchange's V, Vb, probabilities and exterior forces computed in
+ * __float128 (pimdb_oracle_quad.c): the exact values, to about 1e-30, for the
+ * given double inputs; outputs rounded to double, any may be NULL. */
+int orc_exchange_quad(int N, int D, double beta_t, double k, int pbc, double L,
+                      const double* x1, const double* x2, const double* xPm1, const double* xP,
+                      double* V_out, double* Vb_out, double* prob_out, double* f_first, double* f_last);
 /* Brute force: V_B summed over all 2^(N-1) compositions of 0..N-1 into
  * consecutive cycles, -1/beta_t ln sum_c prod_blocks exp(-beta_t E)/(e+1). */
 double orc_vb_enumerate(int N, double beta_t, const double* E_kn);

@@ -1,12 +1,15 @@
 """Writes the golden fixtures under tests/golden/ from the reference's own
 regression data (higj/pimd-b tests/cases/<case>/).
 
-The fixtures are data only: the case .ini inputs and the first frames of the
-expected per-bead trajectories (positions in angstrom, velocities in
-angstrom/ps, forces in eV/angstrom, as the reference prints them,
-src/states/{position,velocity,force}.cpp) and of output/simulation.out
-(src/observables/observable.cpp:61-116).  Run once in a container that has the
-reference tree:
+The fixtures are data only: the case .ini inputs, the expected per-bead
+trajectories (positions in angstrom, velocities in angstrom/ps, forces in
+eV/angstrom, as the reference prints them, src/states/{position,velocity,
+force}.cpp) and output/simulation.out (src/observables/observable.cpp:61-116).
+The first NFRAMES frames and NROWS rows are kept in float64 (compared at 1e-9
+and below); every stored frame is also kept in float32 (`*_all`, compared at
+the reference harness's rtol 1e-5, tests/main.py:77-86, which float32's 6e-8
+rounding leaves intact), and every simulation.out row in float64
+(`sim_out_all`).  Run once in a container that has the reference tree:
 
     python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -64,13 +67,16 @@ def main(ref):
         src = os.path.join(cases, case)
         shutil.copyfile(os.path.join(src, case + ".ini"), os.path.join(HERE, "cases", case + ".ini"))
         header, rows = read_sim_out(os.path.join(src, "simulation.out"), NROWS)
-        data = dict(sim_out_header=np.array(header), sim_out=rows)
+        _, all_rows = read_sim_out(os.path.join(src, "simulation.out"), 10 ** 9)
+        data = dict(sim_out_header=np.array(header), sim_out=rows, sim_out_all=all_rows)
         if case in DYNAMICS:
             nbeads = len([f for f in os.listdir(src) if f.startswith("position_")])
             for name, stem in (("positions", "position_%d.xyz"), ("velocities", "velocity_%d.dat"),
                                ("forces", "force_%d.dat")):
-                data[name] = np.stack([read_frames(os.path.join(src, stem % b), NFRAMES)
-                                       for b in range(nbeads)], axis=1)
+                full = np.stack([read_frames(os.path.join(src, stem % b), 10 ** 9)
+                                 for b in range(nbeads)], axis=1)
+                data[name] = full[:NFRAMES].copy()
+                data[name + "_all"] = full.astype(np.float32)
         np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
         print("wrote", case, {k: v.shape for k, v in data.items()})
 

@@ -90,6 +90,57 @@ def test_observables_match_simulation_out(case):
             assert abs(ob[NAMES.index(name)] - ref) <= 2e-8 * abs(ref) + 1e-8, (name, ob, ref)
 
 
+def case_steps(case):
+    """The run length (the [simulation] steps key) of a case."""
+    import configparser
+    from helpers import GOLDEN
+    cp = configparser.ConfigParser(inline_comment_prefixes=(";", "#"))
+    cp.read(os.path.join(GOLDEN, "cases", case + ".ini"))
+    return int(float(cp["simulation"]["steps"]))
+
+
+@pytest.mark.parametrize("case", OUT)
+def test_full_case_at_harness_tolerance(case):
+    """The whole case from the reference's initial state with its RanMars
+    noise: every stored frame (positions, velocities, forces of every bead,
+    printed before step k sfreq) and every simulation.out row (row s after
+    s + 1 steps: the loop runs steps + 1 iterations, simulation.cpp:236-279)
+    within the reference harness's test, np.allclose(rtol=1e-5)
+    (tests/main.py:77-86)."""
+    kw, sfreq = case_params(case)
+    fx = load_fixture(case)
+    header = [str(h) for h in fx["sim_out_header"]]
+    rows = fx["sim_out_all"]
+    dyn = "positions_all" in fx.files
+    sim = O.Sim(O.make_params(**kw))
+    sim.init_random()
+    steps = case_steps(case)
+    got_rows = []
+    frames = {"positions": [], "velocities": [], "forces": []}
+    ri = 0
+    for s in range(steps + 1):
+        if dyn and s % sfreq == 0:
+            x, p, f = sim.state()
+            frames["positions"].append(x / ANGSTROM)
+            frames["velocities"].append(p / kw["mass"] / VEL_A_PS)
+            frames["forces"].append(f / FORCE_EV_A)
+        sim.step()
+        if ri < rows.shape[0] and int(rows[ri][0]) == s:
+            got_rows.append(sim.observables() / KELVIN)
+            ri += 1
+    assert ri == rows.shape[0]
+    got_rows = np.array(got_rows)
+    for j, name in enumerate(header[1:]):
+        if name in ("ext_pot", "int_pot"):
+            continue
+        assert np.allclose(got_rows[:, NAMES.index(name)], rows[:, 1 + j], rtol=1e-5), name
+    if dyn:
+        for key, got in frames.items():
+            want = fx[key + "_all"]
+            assert len(got) == want.shape[0]
+            assert np.allclose(np.array(got), want, rtol=1e-5), key
+
+
 REF_LIB = os.path.join(os.path.dirname(O.__file__), "_ref", "libpimdb_ref.so")
 
 

@@ -457,3 +457,38 @@ def test_nose_hoover_conserves_extended_energy(kind, nm):
         drifts.append(emax / abs(e0))
     assert drifts[0] < 1e-3, drifts
     assert drifts[1] < 0.4 * drifts[0], drifts
+
+
+@pytest.mark.parametrize("N,pbc", [(1, 0), (2, 1), (5, 0), (8, 1)])
+def test_quad_exchange_pins(N, pbc):
+    """oracle.exchange_quad (the __float128 restatement used as the exact
+    value at N=1024): agrees with the double oracle to its rounding, its V_B
+    with the composition enumeration, and its exterior forces (with the
+    interior links taken out) with central differences of its own V_B."""
+    rng = np.random.default_rng(40 + N)
+    L, beta, k = 6.0, 1.1, 0.7
+    x1 = rng.uniform(-2, 2, size=(N, 3))
+    xP = x1 + rng.normal(0, 0.6, size=(N, 3))
+    q = O.exchange_quad(x1, x1, xP, xP, beta, k, pbc, L)
+    o = O.exchange(x1, x1, xP, xP, beta, k, pbc, L)
+    for key in ("V", "Vb", "f_first", "f_last"):
+        assert np.max(np.abs(q[key] - o[key])) <= 1e-13 * max(1.0, np.max(np.abs(q[key]))), key
+    assert np.max(np.abs(q["prob"] - o["prob"])) <= 1e-13
+    E = O.cycle_energies(x1, xP, k, pbc, L)
+    assert abs(q["V"][N] - O.vb_enumerate(E, N, beta)) <= 1e-12 * max(1.0, abs(q["V"][N]))
+    h = 1e-6
+    for which, xs in ((0, x1), (1, xP)):
+        for i in range(N):
+            for a in range(3):
+                xp, xm = xs.copy(), xs.copy()
+                xp[i, a] += h
+                xm[i, a] -= h
+                if which == 0:
+                    vp = O.exchange_quad(xp, xp, xP, xP, beta, k, pbc, L)["V"][N]
+                    vm = O.exchange_quad(xm, xm, xP, xP, beta, k, pbc, L)["V"][N]
+                    got = q["f_first"][i, a]
+                else:
+                    vp = O.exchange_quad(x1, x1, xp, xp, beta, k, pbc, L)["V"][N]
+                    vm = O.exchange_quad(x1, x1, xm, xm, beta, k, pbc, L)["V"][N]
+                    got = q["f_last"][i, a]
+                assert abs(got + (vp - vm) / (2 * h)) < 1e-7, (which, i, a)
