@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed batched MD steps")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--run-length", type=int, default=2000,

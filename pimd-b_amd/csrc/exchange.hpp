@@ -43,9 +43,12 @@ constexpr int REC_B = 64;   // targets per block: one per lane of the sequential
 constexpr int REC_MAXWAVES = 16;
 constexpr int REC_UNROLL = 8;  // independent source loads in flight per lane
 constexpr int REC_PF = 16;     // in-block entries read ahead by the sequential passes
-// Row stride of the in-block matrix: odd, so that a wave reading down a column
-// (the in-block triangle passes) spreads over the LDS banks.
-constexpr int REC_LD = REC_B + 1;
+// The in-block matrix holds its strict lower triangle only, packed by
+// target: entry (j, t), j < t, at rec_idx(j, t) = t (t - 1) / 2 + j, so that
+// a wave writing one target's sources (the triangle passes) writes
+// consecutive doubles.
+constexpr int REC_TRI = REC_B * (REC_B - 1) / 2;
+__host__ __device__ __forceinline__ int rec_idx(int j, int t) { return t * (t - 1) / 2 + j; }
 
 // Row u of the row-major copy ET starts at u N - u(u-1)/2 and holds v = u..N-1.
 __host__ __device__ __forceinline__ size_t et_row(int u, int N) { return (size_t)u * N - (size_t)u * (u - 1) / 2; }
@@ -216,36 +219,39 @@ __device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, doub
 // Shared-memory workspace of one workgroup running one recursion.
 struct RecShared {
     double* U;         // [N+1]
-    double* lg;        // [N+1]  ln(i)
+    double* lg;        // [N+1]  ln(i), shared by both directions (staged with the ring)
     double* R;         // [REC_B] shifts of the block's targets
+    double* z;         // [REC_B] the block's forward-substitution results
     double* part;      // [nw][REC_B] per-wave partial min of the earlier sources
     double* psum;      // [nw][REC_B] per-wave partial sums scaled to that min, then to R
-    double* a;         // [REC_B*REC_LD] in-block weights, then coefficients, a[j*REC_LD + t]
+    double* a;         // [REC_TRI] in-block weights, then coefficients, at rec_idx(j, t)
     double* x1;        // [D][N] bead 1
     double* xP;        // [D][N] bead P
     double* Shi;       // [N] link prefix sums
     double* Slo;       // [N]
 };
 
-// Doubles of the staged ring (beads 1 and P, split prefix sums) and of one
-// direction's recursion workspace for a group of nw waves.
+// Doubles of the staged ring (beads 1 and P, split prefix sums), of the
+// recursions' shared table ln i (after the ring: the exterior forces' scratch
+// may overlay it) and of one direction's workspace for a group of nw waves.
 __host__ __device__ inline size_t rec_ring_doubles(int N, int D) { return (size_t)(2 * D + 2) * N; }
+__host__ __device__ inline size_t rec_lg_doubles(int N) { return (size_t)N + 1; }
 __host__ __device__ inline size_t rec_dir_doubles(int N, int nw) {
-    return 2 * (size_t)(N + 1) + REC_B + 2 * (size_t)nw * REC_B + (size_t)REC_B * REC_LD;
+    return (size_t)(N + 1) + 2 * REC_B + 2 * (size_t)nw * REC_B + (size_t)REC_TRI;
 }
-// One workgroup of nw waves running one recursion at a time: the ring, then
-// one workspace.
+// One workgroup of nw waves running one recursion at a time: the ring, ln i,
+// then one workspace.
 __host__ __device__ inline size_t rec_shared_doubles(int N, int D, int nw) {
-    return rec_ring_doubles(N, D) + rec_dir_doubles(N, nw);
+    return rec_ring_doubles(N, D) + rec_lg_doubles(N) + rec_dir_doubles(N, nw);
 }
 
 __device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N, int nw) {
     s.U = p; p += N + 1;
-    s.lg = p; p += N + 1;
     s.R = p; p += REC_B;
+    s.z = p; p += REC_B;
     s.part = p; p += (size_t)nw * REC_B;
     s.psum = p; p += (size_t)nw * REC_B;
-    s.a = p; p += (size_t)REC_B * REC_LD;
+    s.a = p; p += (size_t)REC_TRI;
 }
 
 // Ring first, so that after the recursions everything from
@@ -258,6 +264,7 @@ __device__ __forceinline__ RecShared rec_carve(double* base, int N, int D, int n
     s.xP = p; p += (size_t)D * N;
     s.Shi = p; p += N;
     s.Slo = p; p += N;
+    s.lg = p; p += N + 1;
     rec_carve_dir(s, p, N, nw);
     return s;
 }
@@ -286,6 +293,7 @@ __device__ __forceinline__ Ring ring_stage(const RecShared& sh, int N, const dou
         sh.x1[e] = x1[e];
         sh.xP[e] = xP[e];
     }
+    for (int i = threadIdx.x; i <= N; i += blockDim.x) sh.lg[i] = i > 0 ? log((double)i) : 0.0;
     __syncthreads();
     const Ring r = make_ring(sh.x1, sh.xP, sh.Shi, sh.Slo, k, pbc, L);
     if (threadIdx.x < WAVE) {
@@ -357,35 +365,31 @@ __device__ __forceinline__ void rec_tri(int vr, int lane, int& t, int& j) {
 // v_t: for j = 0 .. REC_B-2 in turn, every lane takes
 //     v_t = min(v_t, a[j][t] + v_j)   (KIND 0, the min-plus pass) or
 //     v_t = v_t + a[j][t] v_j         (KIND 1, the forward substitution),
-// v_j being what lane j holds at step j (final by then: a[j'][t] is +inf,
-// resp. 0, for j' >= t).  Straight-line code, with a[j][t] read REC_PF steps
-// ahead, so each step costs one cross-lane read and one or two operations.
-// Targets past the end of a short block only meet +inf / 0 entries.
+// v_j being what lane j holds at step j (final by then).  Target t's result is
+// its value at step t: the broadcast v_j is stored to out[j] as it is taken
+// (out[REC_B-1] from the last lane after the last step).  After its own step a
+// lane's value is no longer used, so the pass reads column t of the packed
+// triangle straight on past its end (into the next columns: finite entries
+// or +inf, in bounds for every t) instead of selecting +inf / 0 there.
+// Straight-line code, with a[j][t] read REC_PF steps ahead, so each step
+// costs one cross-lane read and one or two operations on the dependent chain.
+// The results are in out[] for the whole wave on return.
 template <int KIND>
-__device__ __forceinline__ double rec_inblock_seq(const double* a, int t, double v) {
+__device__ __forceinline__ void rec_inblock_seq(const double* a, int t, double v, double* out) {
+    const int base = rec_idx(0, t);   // rec_idx(REC_B - 2, REC_B - 1) = REC_TRI - 1 bounds every read
     double buf[REC_PF];
 #pragma unroll
-    for (int u = 0; u < REC_PF; ++u) buf[u] = a[u * REC_LD + t];
+    for (int u = 0; u < REC_PF; ++u) buf[u] = a[base + u];
 #pragma unroll
     for (int j = 0; j < REC_B - 1; ++j) {
         const double aj = buf[j % REC_PF];
-        if (j + REC_PF < REC_B) buf[j % REC_PF] = a[(j + REC_PF) * REC_LD + t];
+        if (j + REC_PF < REC_B - 1) buf[j % REC_PF] = a[base + j + REC_PF];
         const double vj = wave_bcast_uniform(v, j);
+        out[j] = vj;
         v = KIND == 0 ? fmin(v, aj + vj) : fma(aj, vj, v);
     }
-    return v;
-}
-
-// a[j][t] = v for j >= t (the upper triangle and the diagonal): the strict
-// upper triangle is the transpose of rec_tri's, the diagonal one entry per
-// lane.  Run by the group's nw waves.
-__device__ __forceinline__ void rec_fill_upper(double* a, int wave, int lane, int nw, double v) {
-    for (int vr = wave; vr < REC_B / 2; vr += nw) {
-        int t, j;
-        rec_tri(vr, lane, t, j);
-        if (t >= 0) a[t * REC_LD + j] = v;
-    }
-    if (wave == 0) a[lane * REC_LD + lane] = v;
+    if (t == REC_B - 1) out[REC_B - 1] = v;
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Blocks of the recursions for N particles; the backward recursion has N-1
@@ -412,10 +416,7 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
     // the partials are sized for the group's waves (rec_carve)
     const int nwave = nthr / WAVE;
     const int ntarget = DIR == 0 ? N : N - 1;
-    for (int i = tid; i <= N; i += nthr) {
-        sh.lg[i] = (i > 0) ? log((double)i) : 0.0;
-        sh.U[i] = 0.0;
-    }
+    for (int i = tid; i <= N; i += nthr) sh.U[i] = 0.0;   // (ln i is staged with the ring)
     __syncthreads();
     const int nblk = rec_blocks(N);
     for (int blk = 0; blk < nblk; ++blk) {
@@ -431,18 +432,18 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         const int tn = n0 + min(lane, nb - 1);   // the lane's target (clamped)
         const RecTarget<D> tg = rec_target<DIR, D>(sh, N, tn);
         // (0) in-block weights w(n0+t, n0+j), j < t < nb, staged as a[j][t]
-        // (the strict lower triangle, two rows per wave pass); +inf elsewhere
+        // (the strict lower triangle, two rows per wave pass); +inf for
+        // targets past the block's end
         for (int vr = wave; vr < REC_B / 2; vr += nthr / WAVE) {
             int t, j;
             rec_tri(vr, lane, t, j);
             if (t >= 0 && t < nb) {
                 const RecTarget<D> tt = rec_target<DIR, D>(sh, N, n0 + t);
-                sh.a[j * REC_LD + t] = rec_w<DIR, D>(sh, r, bt, N, tt, n0 + j);
+                sh.a[rec_idx(j, t)] = rec_w<DIR, D>(sh, r, bt, N, tt, n0 + j);
             } else if (t >= nb) {
-                sh.a[j * REC_LD + t] = INFINITY;
+                sh.a[rec_idx(j, t)] = INFINITY;
             }
         }
-        rec_fill_upper(sh.a, wave, lane, nthr / WAVE, INFINITY);
         // (1) one pass over this wave's share of earlier sources: the lane's
         // running minimum m of y_q = w(n,q) + U_q and s = sum_q exp(-(y_q - m)),
         // rescaled whenever a chunk lowers m
@@ -461,15 +462,15 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
 #pragma unroll
                     for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
                     const double mn = fmin(m, cm);
-                    s *= exp_minimax(mn - m);   // m = inf (first chunk): s = 0 stays 0
+                    s *= exp_pair(mn - m);   // m = inf (first chunk): s = 0 stays 0
                     m = mn;
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) s += exp_minimax(mn - y[u]);
+                    for (int u = 0; u < UNROLL; ++u) s += exp_pair(mn - y[u]);
                 }
                 for (; q < qhi; ++q) {
                     const double y = rec_w<DIR, D>(sh, r, bt, N, tg, q) + sh.U[q];
                     const double mn = fmin(m, y);
-                    s = fma(s, exp_minimax(mn - m), exp_minimax(mn - y));
+                    s = fma(s, exp_pair(mn - m), exp_pair(mn - y));
                     m = mn;
                 }
             }
@@ -483,7 +484,7 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             double rr = INFINITY;
             for (int w = 0; w < nwave; ++w) rr = fmin(rr, sh.part[w * REC_B + t]);
             if (t >= nb) rr = INFINITY;
-            sh.R[t] = rec_inblock_seq<0>(sh.a, t, rr);
+            rec_inblock_seq<0>(sh.a, t, rr, sh.R);
         }
         __syncthreads();
         // (2a) each wave's earlier-source sum relative to R_t,
@@ -492,7 +493,7 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             const int t = lane;
             const double sw = sh.psum[wave * REC_B + t];
             double c = 0.0;
-            if (t < nb && sw > 0.0) c = sw * exp_minimax(sh.R[t] - sh.part[wave * REC_B + t]);
+            if (t < nb && sw > 0.0) c = sw * exp_pair(sh.R[t] - sh.part[wave * REC_B + t]);
             sh.psum[wave * REC_B + t] = c;
         }
         // (2b) in-block coefficients a[j][t] = exp(-(w(n0+t, n0+j) + R_j - R_t)),
@@ -500,10 +501,9 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
         for (int vr = wave; vr < REC_B / 2; vr += nthr / WAVE) {
             int t, j;
             rec_tri(vr, lane, t, j);
-            if (t >= 0 && t < nb) sh.a[j * REC_LD + t] = exp_minimax(-(sh.a[j * REC_LD + t] + sh.R[j] - sh.R[t]));
-            else if (t >= nb) sh.a[j * REC_LD + t] = 0.0;
+            if (t >= 0 && t < nb) sh.a[rec_idx(j, t)] = exp_pair(-(sh.a[rec_idx(j, t)] + sh.R[j] - sh.R[t]));
+            else if (t >= nb) sh.a[rec_idx(j, t)] = 0.0;
         }
-        rec_fill_upper(sh.a, wave, lane, nthr / WAVE, 0.0);
         __syncthreads();
         // (2c) in-block forward substitution on wave 0 (a[j][t] = 0 for
         // j >= t), then U = R - ln z
@@ -511,7 +511,8 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
             const int t = lane;
             double z0 = 0.0;
             for (int w = 0; w < nwave; ++w) z0 += sh.psum[w * REC_B + t];
-            const double z = rec_inblock_seq<1>(sh.a, t, z0);
+            rec_inblock_seq<1>(sh.a, t, z0, sh.z);
+            const double z = sh.z[t];
             if (t < nb) {
                 const double u = sh.R[t] - log(z);
                 sh.U[n0 + t] = u;
@@ -766,7 +767,7 @@ __device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, c
                 for (int ax = 0; ax < D; ++ax) d[ax] = ring_image(rp[ax] - xa[ax], r);
                 // -beta~ (ecyc(a, b) + V[a] + Vb[b+1] - V[N])
                 const double arg = fma(g, ring_sdiff(rp[D], rp[D + 1], sha, sla) + norm2_fma<D>(d), ca + rp[D + 2]);
-                double pr = rp[D + 3] * exp_minimax(arg);
+                double pr = rp[D + 3] * exp_pair(arg);
                 if (masked) pr = ok ? pr : 0.0;
 #pragma unroll
                 for (int ax = 0; ax < D; ++ax) {

@@ -109,7 +109,8 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
                 return true;
             };
             int slots = 0;
-            const size_t dual = 2 * rec_dir_doubles(N, nw / 2), single = rec_dir_doubles(N, nw);
+            const size_t lg = rec_lg_doubles(N);
+            const size_t dual = lg + 2 * rec_dir_doubles(N, nw / 2), single = lg + rec_dir_doubles(N, nw);
             if (nw >= 2 && fits(dual, std::min(smax, 4), slots)) {
                 d.xdual = 1;
                 need = std::max(need, ring + std::max(dual, fix + (size_t)slots * slot));
