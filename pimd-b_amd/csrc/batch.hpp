@@ -5,8 +5,8 @@
 // OBABO step of a replica with workgroup barriers only:
 //
 //   O-step (+ zeroMomentum) -> B + A -> pair/ext/spring forces of all beads
-//   -> cycle energies -> forward and backward recursions -> exchange forces
-//   on beads 1 and P -> B -> O-step (+ zeroMomentum)
+//   -> forward recursion -> backward recursion with the exchange forces on
+//   beads 1 and P -> B -> O-step (+ zeroMomentum)
 //
 // (Simulation::run, src/simulation.cpp:246-259).  The device noise uses the
 // same Philox keys as the single-system form, so a one-replica batch follows
@@ -14,7 +14,7 @@
 #pragma once
 
 #include "device_common.hpp"
-#include "exchange.hpp"
+#include "exchange_batch.hpp"
 #include "pairs.hpp"
 
 namespace pimdb {
@@ -22,22 +22,21 @@ namespace pimdb {
 struct RepDesc {
     int N, P, bosonic, pbc;
     int thermostat, nm, fixcom;
-    int xdual;                  // exchange recursions side by side (LDS permitting)
     int plds;                   // pair forces accumulated in LDS (LDS permitting)
     int fused;                  // normal-mode O-step on the matrix cores with B + A fused in (see wg_momenta)
-    int xslots;                 // exterior-force chunk slots that the workgroup's LDS holds (>= 1)
+    int xslots;                 // exchange chunk slots that the workgroup's LDS holds (1 .. waves)
     double L, invL, mass, dt;
     double k, bt, c1, c2;
     double int_k, strength, cutoff, cut2;
     ExtPot ext;
     uint32_t seed, rid;
     unsigned long long off_x;   // P*D*N block of x, p, f
-    unsigned long long off_V;   // N+1 block of V, Vb
+    unsigned long long off_V;   // N+1 block of V
     unsigned long long off_C;   // P*P block of Ct, CTt
 };
 
 struct BatchPtrs {
-    double *x, *p, *f, *V, *Vb;
+    double *x, *p, *f, *V;
     const double *Ct, *CTt;
     const RepDesc* reps;
     const int* sched;          // replicas of workgroup g: sched[sched_off[g] .. sched_off[g+1])
@@ -58,10 +57,6 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
     for (int w = 0; w < nw; ++w) s += red[w];
     return s;
 }
-
-#ifndef BATCH_REC_UNROLL
-#define BATCH_REC_UNROLL 4
-#endif
 
 // Wave-uniform copies (v_readfirstlane) of values that reach a non-inlined
 // function in vector registers.
@@ -86,7 +81,7 @@ __device__ __forceinline__ T* uni(T* p) {
 __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
-    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom); u.xdual = uni(r.xdual);
+    u.thermostat = uni(r.thermostat); u.nm = uni(r.nm); u.fixcom = uni(r.fixcom);
     u.plds = uni(r.plds); u.fused = uni(r.fused); u.xslots = uni(r.xslots);
     u.L = uni(r.L); u.invL = uni(r.invL); u.mass = uni(r.mass); u.dt = uni(r.dt);
     u.k = uni(r.k); u.bt = uni(r.bt); u.c1 = uni(r.c1); u.c2 = uni(r.c2);
@@ -688,49 +683,23 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
     return P <= NM_MFMA_MAXP ? nm_mfma_lds_doubles(P, nwaves) : 2 * (size_t)P * WAVE;
 }
 
-// Exchange of one replica: cycle energies, both recursions, exterior forces
-// added into f of beads 1 and P.  Kept out of line (its registers would
-// otherwise add to every other phase's); the arguments are made wave-uniform
-// on entry.
+// Exchange of one replica (exchange_batch.hpp): the forward recursion, then
+// the backward recursion in linear form with the exterior forces added into f
+// of beads 1 and P.  Kept out of line (its registers would otherwise add to
+// every other phase's); the arguments are made wave-uniform on entry.
 template <int D>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
-                                         double* f_, double* V_, double* Vb_, int* status_, double* lds_, int dual_,
-                                         int slots_) {
+                                         double* f_, double* V_, int* status_, double* lds_, int slots_) {
     const int N = uni(N_), P = uni(P_);
-    const bool dual = uni(dual_) != 0;
     const int slots = uni(slots_);
-    const bool pbc = uni(pbc_) != 0;
     const double L = uni(L_), k = uni(k_), bt = uni(bt_);
     const double* x = uni(x_);
     double* f = uni(f_);
     double* V = uni(V_);
-    double* Vb = uni(Vb_);
     int* status = uni(status_);
     double* lds = uni(lds_);
-    const size_t bs = (size_t)D * N;
-    // beads 1 and P and the link prefix sums in LDS; cycle energies are
-    // recomputed from them wherever they are needed
-    const double* x2 = x + bs * (1 % P);
-    const double* xPm1 = x + bs * ((P - 2 + P) % P);
-    if (dual) {
-        // the two recursions side by side: even waves forward, odd waves
-        // backward (their in-block chains then sit on different SIMDs)
-        const int wave = uni((int)(threadIdx.x / WAVE)), nw = blockDim.x / WAVE;
-        const RecDual rd = rec_carve_dual(lds, N, D, nw / 2);
-        const Ring r = ring_stage<D>(rd.s0, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
-        const int gtid = (wave >> 1) * WAVE + (int)(threadIdx.x % WAVE), gnthr = (nw / 2) * WAVE;
-        if ((wave & 1) == 0) exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s0, gtid, gnthr);
-        else exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, rd.s1, gtid, gnthr);
-        __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D), slots);
-    } else {
-        const RecShared sh = rec_carve(lds, N, D, blockDim.x / WAVE);
-        const Ring r = ring_stage<D>(sh, N, x, x + bs * (P - 1), k, pbc, L, nullptr, nullptr);
-        exchange_recursion<0, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
-        exchange_recursion<1, D, BATCH_REC_UNROLL>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
-        __threadfence_block();
-        wg_exterior_forces<D>(N, bt, k, r, V, Vb, x2, xPm1, f, f + bs * (P - 1), lds + rec_ring_doubles(N, D), slots);
-    }
+    if (uni(pbc_) != 0) xb_exchange<D, true>(N, P, L, k, bt, x, f, V, status, lds, slots);
+    else xb_exchange<D, false>(N, P, L, k, bt, x, f, V, status, lds, slots);
 }
 
 // Work of one launch: a full MD step, or parts of it for timing.
@@ -759,7 +728,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.Vb + r.off_V, bp.status + ri, lds, r.xdual, r.xslots);
+                           bp.status + ri, lds, r.xslots);
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];

@@ -81,9 +81,8 @@ static void build_schedule(pimdb_batch* b, std::vector<int>& sched, std::vector<
 
 // LDS plan of every replica for `threads` threads per workgroup and
 // `per_cu` workgroups per compute unit (each then gets 1/per_cu of its LDS):
-// which phases keep their work in LDS (the exchange recursions side by side,
-// the pair forces of every wave's slice), how many chunk slots the exterior
-// forces get, and the largest need.  Returns the dynamic LDS bytes, or 0
+// whether the pair forces of every wave's slice stay in LDS, how many chunk
+// slots the exchange's backward pass gets, and the largest need.  Returns the dynamic LDS bytes, or 0
 // (with *why) when some replica does not fit.
 static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepDesc>& desc, std::string* why) {
     const size_t budget = device_lds_limit() / (size_t)per_cu / sizeof(double);   // doubles
@@ -94,34 +93,25 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
         const pimdb_params& p = b->prm[r];
         const int N = p.natoms, D = p.ndim;
         RepDesc& d = desc[r];
-        d.xdual = 0;
         d.xslots = 1;
         if (d.bosonic) {
-            // recursion workspace, and after it (over the same LDS) the
-            // exterior forces' scratch with as many chunk slots as fit
-            const size_t ring = rec_ring_doubles(N, D);
-            const size_t fix = ext_fixed_doubles(N, D), slot = ext_slot_doubles(D);
-            const int smax = ext_slots(N, nw, EXT_SLOTS);
-            auto fits = [&](size_t rec, int min_slots, int& slots) {
-                if (ring + rec > avail || ring + fix + (size_t)min_slots * slot > avail) return false;
-                const size_t room = (avail - ring - fix) / slot;
-                slots = (int)std::min<size_t>((size_t)smax, room);
-                return true;
-            };
-            int slots = 0;
-            const size_t lg = rec_lg_doubles(N);
-            const size_t dual = lg + 2 * rec_dir_doubles(N, nw / 2), single = lg + rec_dir_doubles(N, nw);
-            if (nw >= 2 && fits(dual, std::min(smax, 4), slots)) {
-                d.xdual = 1;
-                need = std::max(need, ring + std::max(dual, fix + (size_t)slots * slot));
-            } else if (fits(single, 1, slots)) {
-                need = std::max(need, ring + std::max(single, fix + (size_t)slots * slot));
-            } else {
+            // the exchange's layout with as many chunk slots (up to one per
+            // wave) as fit
+            if (N > XB_MAXPER * threads) {
+                *why = "replica " + std::to_string(r) + ": natoms above " + std::to_string(XB_MAXPER) +
+                       " per thread of a batched workgroup";
+                return 0;
+            }
+            int slots = nw;
+            size_t x = (size_t)xb_layout(N, D, nw, slots).total;
+            while (slots > 1 && x > avail) x = (size_t)xb_layout(N, D, nw, --slots).total;
+            if (x > avail) {
                 *why = "replica " + std::to_string(r) + ": natoms too large for the exchange workspace of a "
                        "batched workgroup";
                 return 0;
             }
             d.xslots = slots;
+            need = std::max(need, x);
         }
         const size_t nm = batch_nm_lds_doubles(p.nbeads, nw);
         if (nm > avail) {
@@ -317,7 +307,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     };
     double *Ct = nullptr, *CTt = nullptr;
     alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
-    alloc(&b->bp.V, b->nV); alloc(&b->bp.Vb, b->nV);
+    alloc(&b->bp.V, b->nV);
     alloc(&b->bp.cmp, 8 * (size_t)nrep);
     alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
     b->bp.Ct = Ct;
@@ -361,7 +351,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->s) (void)hipStreamSynchronize(b->s);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.Vb, b->bp.cmp,
+    double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.cmp,
                       const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
