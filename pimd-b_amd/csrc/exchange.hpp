@@ -219,7 +219,7 @@ __device__ __forceinline__ void ring_prefix(const Ring& r, int N, int lane, doub
 // Shared-memory workspace of one workgroup running one recursion.
 struct RecShared {
     double* U;         // [N+1]
-    double* lg;        // [N+1]  ln(i), shared by both directions (staged with the ring)
+    double* lg;        // [N+1]  ln(i) (staged with the ring)
     double* R;         // [REC_B] shifts of the block's targets
     double* z;         // [REC_B] the block's forward-substitution results
     double* part;      // [nw][REC_B] per-wave partial min of the earlier sources
@@ -232,8 +232,7 @@ struct RecShared {
 };
 
 // Doubles of the staged ring (beads 1 and P, split prefix sums), of the
-// recursions' shared table ln i (after the ring: the exterior forces' scratch
-// may overlay it) and of one direction's workspace for a group of nw waves.
+// table ln i and of one direction's workspace for a group of nw waves.
 __host__ __device__ inline size_t rec_ring_doubles(int N, int D) { return (size_t)(2 * D + 2) * N; }
 __host__ __device__ inline size_t rec_lg_doubles(int N) { return (size_t)N + 1; }
 __host__ __device__ inline size_t rec_dir_doubles(int N, int nw) {
@@ -254,9 +253,8 @@ __device__ __forceinline__ void rec_carve_dir(RecShared& s, double*& p, int N, i
     s.a = p; p += (size_t)REC_TRI;
 }
 
-// Ring first, so that after the recursions everything from
-// base + rec_ring_doubles(N, D) on is free (the exterior forces' scratch).
-// nw: the waves of the group that will run the recursion.
+// The ring, ln i, then the workspace; nw: the waves of the group that will
+// run the recursion.
 __device__ __forceinline__ RecShared rec_carve(double* base, int N, int D, int nw) {
     RecShared s;
     double* p = base;
@@ -267,21 +265,6 @@ __device__ __forceinline__ RecShared rec_carve(double* base, int N, int D, int n
     s.lg = p; p += N + 1;
     rec_carve_dir(s, p, N, nw);
     return s;
-}
-
-// Workspace of the two recursions run side by side by two halves of one
-// workgroup (the batched form), nw waves each: the staged ring, then each
-// direction's own U, ln i, R, partials and in-block matrix.
-struct RecDual {
-    RecShared s0, s1;
-};
-__device__ __forceinline__ RecDual rec_carve_dual(double* base, int N, int D, int nw) {
-    RecDual rd;
-    rd.s0 = rec_carve(base, N, D, nw);
-    double* p = base + rec_shared_doubles(N, D, nw);
-    rd.s1 = rd.s0;
-    rec_carve_dir(rd.s1, p, N, nw);
-    return rd;
 }
 
 // Copies beads 1 and P into the workspace and builds the prefix sums there
@@ -633,211 +616,6 @@ __device__ __forceinline__ void exterior_force(int which, int l, int N, int lane
         if (!grad_only) s += ring_image(link[a * N + l] - xs[a], r);
         F[a] = s * k;
     }
-}
-
-// The exterior spring forces of beads 1 and P for one ring on a whole
-// workgroup (the batched form): the sums of exterior_force with every
-// connection probability evaluated once.  P[b][a] with a <= b (the cycle
-// through particles a..b, closed by the last bead of b and the first bead of
-// a) enters both
-//     F_1[a] += k P[b][a] d   (:188-215)   and   F_P[b] -= k P[b][a] d   (:159-186)
-// with d = xP[b] - x1[a] (min image; the bead-P term's displacement is -d).
-// The pairs a <= b are cut into 64 x 64 tiles (A, B), A <= B, and each tile's
-// 64 lane offsets into chunks of `ch` offsets.  A wave takes one chunk at a
-// time: lane t holds particle a = 64A + t of bead 1 in registers and meets
-// b = 64B + (t + s) mod 64 at offset s, whose data it reads from LDS; its F_1
-// partial stays in registers and the F_P partial of its column moves one lane
-// along the wave after every offset (DPP), so every lane adds to its own
-// column.  In a diagonal tile only b >= a counts.  Each chunk leaves its
-// partials in its own LDS slot; after a round of up to EXT_SLOTS chunks they
-// are added per particle in chunk order, so the result depends on the data
-// only.  The links b = a - 1 (probability 1 - exp(...)) and the interior links
-// are added per particle at the end: f[bead][a][l] += k (S + links).
-// Scratch (doubles): ext_scratch_doubles(N, D, blockDim.x, slot_cap).
-constexpr int EXT_SLOTS = 32;
-
-// Chunk length: the longest of 64, 32, 16, 8 offsets that gives every wave a
-// chunk.
-__host__ __device__ inline int ext_chunk(int N, int nw) {
-    const int T = (N + WAVE - 1) / WAVE, ntile = T * (T + 1) / 2;
-    int ch = WAVE;
-    while (ch > 8 && ntile * (WAVE / ch) < nw) ch >>= 1;
-    return ch;
-}
-// Chunks per round: every chunk if they fit, at most EXT_SLOTS, and at most
-// `cap` (the slots the caller's LDS holds, >= 1).
-__host__ __device__ inline int ext_slots(int N, int nw, int cap) {
-    const int T = (N + WAVE - 1) / WAVE, ntile = T * (T + 1) / 2;
-    const int nchunk = ntile * (WAVE / ext_chunk(N, nw));
-    int s = nchunk < EXT_SLOTS ? nchunk : EXT_SLOTS;
-    s = s < cap ? s : cap;
-    return s > 1 ? s : 1;
-}
-// Column record of particle b: xP[b] (D), H_b, L_b, -beta~ (Vb[b+1] - V[N]),
-// 1/(b+1); an odd stride, so that lanes reading consecutive records spread
-// over the LDS banks.
-constexpr int EXT_REC = 9;
-// Scratch of the exterior forces: a part that grows with N, plus one slot of
-// chunk partials per chunk of a round.
-__host__ __device__ inline size_t ext_fixed_doubles(int N, int D) {
-    return 2 * (size_t)N + 1 + (size_t)EXT_REC * N + 2 * (size_t)D * N;
-}
-__host__ __device__ inline size_t ext_slot_doubles(int D) { return 2 * (size_t)D * WAVE; }
-__host__ __device__ inline size_t ext_scratch_doubles(int N, int D, int nthreads, int cap) {
-    return ext_fixed_doubles(N, D) + (size_t)ext_slots(N, nthreads / WAVE, cap) * ext_slot_doubles(D);
-}
-
-// Tile (A, B), A <= B, of index `tile` in row-major order over T x T, and back.
-__device__ __forceinline__ void ext_tile(int tile, int T, int& A, int& B) {
-    A = 0;
-    while (tile >= T - A) {
-        tile -= T - A;
-        ++A;
-    }
-    B = A + tile;
-}
-__device__ __forceinline__ int ext_tile_index(int A, int B, int T) { return A * T - A * (A - 1) / 2 + (B - A); }
-
-template <int D>
-__device__ __forceinline__ void wg_exterior_forces(int N, double bt, double k, const Ring& r,
-                                                   const double* __restrict__ V, const double* __restrict__ Vb,
-                                                   const double* __restrict__ x2, const double* __restrict__ xPm1,
-                                                   double* __restrict__ f1, double* __restrict__ fP, double* scratch,
-                                                   int slot_cap) {
-    const int nthr = blockDim.x, tid = threadIdx.x;
-    const int lane = tid % WAVE, wave = __builtin_amdgcn_readfirstlane(tid / WAVE), nw = nthr / WAVE;
-    static_assert(D + 4 <= EXT_REC, "column record");
-    double* Vl = scratch;                      // [N+1] V
-    double* Vbn = Vl + N + 1;                  // [N]   Vb[b+1] - V[N]
-    double* rec = Vbn + N;                     // [N][EXT_REC] column records
-    double* acc = rec + (size_t)EXT_REC * N;   // [2][D][N] sums over the pairs: F_1, F_P (without k)
-    double* part = acc + 2 * (size_t)D * N;    // [slots][2][D][64] one round's chunk partials
-    const double VN = V[N];
-    const double nbt = -bt, g = -bt * r.halfk;
-    for (int i = tid; i <= N; i += nthr) {
-        Vl[i] = V[i];
-        if (i < N) {
-            const double vbn = Vb[i + 1] - VN;
-            Vbn[i] = vbn;
-            double* rp = rec + (size_t)i * EXT_REC;
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) rp[ax] = r.xP[ax * N + i];
-            rp[D] = r.Shi[i];
-            rp[D + 1] = r.Slo[i];
-            rp[D + 2] = nbt * vbn;
-            rp[D + 3] = 1.0 / (i + 1);
-        }
-    }
-    for (int e = tid; e < 2 * D * N; e += nthr) acc[e] = 0.0;
-    __syncthreads();
-    const int T = (N + WAVE - 1) / WAVE;
-    const int ch = ext_chunk(N, nw), per_tile = WAVE / ch;
-    const int nchunk = T * (T + 1) / 2 * per_tile;
-    const int slots = ext_slots(N, nw, slot_cap);
-    for (int c0 = 0; c0 < nchunk; c0 += slots) {
-        const int cend = min(nchunk, c0 + slots);
-        for (int c = c0 + wave; c < cend; c += nw) {   // wave-uniform
-            int A, B;
-            ext_tile(c / per_tile, T, A, B);
-            const int s0 = (c % per_tile) * ch;
-            const int a = A * WAVE + lane;
-            const bool aok = a < N;
-            const int ac = aok ? a : N - 1;
-            double xa[D], F1[D], FPr[D];
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) {
-                xa[ax] = r.x1[ax * N + ac];
-                F1[ax] = 0.0;
-                FPr[ax] = 0.0;
-            }
-            const double sha = r.Shi[ac], sla = r.Slo[ac], ca = nbt * Vl[ac];
-            // P[b][a] d for this lane's pair at offset s; `masked` tiles (the
-            // diagonal, the ragged last column tile) drop b < a and b >= N
-            auto step = [&](int s, bool masked) {
-                const int bl = (lane + s) & (WAVE - 1);
-                int b = B * WAVE + bl;
-                bool ok = true;
-                if (masked) {
-                    ok = aok && b < N && (A != B || bl >= lane);
-                    b = b < N ? b : N - 1;
-                }
-                const double* rp = rec + (size_t)b * EXT_REC;
-                double d[D];
-#pragma unroll
-                for (int ax = 0; ax < D; ++ax) d[ax] = ring_image(rp[ax] - xa[ax], r);
-                // -beta~ (ecyc(a, b) + V[a] + Vb[b+1] - V[N])
-                const double arg = fma(g, ring_sdiff(rp[D], rp[D + 1], sha, sla) + norm2_fma<D>(d), ca + rp[D + 2]);
-                double pr = rp[D + 3] * exp_pair(arg);
-                if (masked) pr = ok ? pr : 0.0;
-#pragma unroll
-                for (int ax = 0; ax < D; ++ax) {
-                    F1[ax] = fma(pr, d[ax], F1[ax]);
-                    FPr[ax] = lane_rotate(fma(-pr, d[ax], FPr[ax]));
-                }
-            };
-            if (A != B && B * WAVE + WAVE <= N) {
-                for (int s = s0; s < s0 + ch; ++s) step(s, false);
-            } else {
-                for (int s = s0; s < s0 + ch; ++s) step(s, true);
-            }
-        // lane t now holds the F_P partial of column (t + s0 + ch) mod 64
-            double* pw = part + (size_t)(c - c0) * 2 * D * WAVE;
-            const int col = (lane + s0 + ch) & (WAVE - 1);
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) {
-                pw[ax * WAVE + lane] = F1[ax];
-                pw[(D + ax) * WAVE + col] = FPr[ax];
-            }
-        }
-        __syncthreads();
-        // this round's partials, added per particle in chunk order: the
-        // chunks of tiles (A, B >= A) for F_1 of row tile A, of tiles
-        // (A <= B, B) for F_P of column tile B
-        for (int e = tid; e < 2 * D * N; e += nthr) {
-            const int which = e / (D * N), rem = e % (D * N);
-            const int ax = rem / N, pt = (rem % N) / WAVE, pl = (rem % N) % WAVE;
-            double sum = acc[e];
-            for (int o = which == 0 ? pt : 0; o < (which == 0 ? T : pt + 1); ++o) {
-                const int tile = which == 0 ? ext_tile_index(pt, o, T) : ext_tile_index(o, pt, T);
-                for (int sub = 0; sub < per_tile; ++sub) {
-                    const int c = tile * per_tile + sub;
-                    if (c >= c0 && c < cend) sum += part[((size_t)(c - c0) * 2 + which) * D * WAVE + ax * WAVE + pl];
-                }
-            }
-            acc[e] = sum;
-        }
-        __syncthreads();
-    }
-    // the links b = a - 1 and the interior links, one particle per thread
-    for (int l = tid; l < N; l += nthr) {
-        double x1l[D], xPl[D], d1[D], dP[D];
-#pragma unroll
-        for (int ax = 0; ax < D; ++ax) {
-            x1l[ax] = r.x1[ax * N + l];
-            xPl[ax] = r.xP[ax * N + l];
-            d1[ax] = 0.0;
-            dP[ax] = 0.0;
-        }
-        double p1 = 0.0, pP = 0.0;
-        if (l >= 1) {   // bead 1 of l from bead P of l-1: P[l-1][l] = 1 - exp(-bt (V[l] + Vb[l] - V[N]))
-            p1 = 1.0 - exp_minimax(nbt * (Vl[l] + Vbn[l - 1]));
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) d1[ax] = ring_image(r.xP[ax * N + l - 1] - x1l[ax], r);
-        }
-        if (l + 1 < N) {   // bead P of l to bead 1 of l+1: P[l][l+1] = 1 - exp(-bt (V[l+1] + Vb[l+1] - V[N]))
-            pP = 1.0 - exp_minimax(nbt * (Vl[l + 1] + Vbn[l]));
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) dP[ax] = ring_image(r.x1[ax * N + l + 1] - xPl[ax], r);
-        }
-#pragma unroll
-        for (int ax = 0; ax < D; ++ax) {
-            const double s1 = fma(p1, d1[ax], acc[ax * N + l]) + ring_image(x2[ax * N + l] - x1l[ax], r);
-            const double sP = fma(pP, dP[ax], acc[(D + ax) * N + l]) + ring_image(xPm1[ax * N + l] - xPl[ax], r);
-            f1[ax * N + l] = s1 * k + f1[ax * N + l];
-            fP[ax * N + l] = sP * k + fP[ax * N + l];
-        }
-    }
-    __syncthreads();
 }
 
 // Adds the exterior spring forces into f for beads 1 and P:

@@ -234,22 +234,35 @@ __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay
         }
         // (1) this wave's share of the earlier sources q < n0 (a balanced
         // contiguous range), in groups of UNROLL (the last group padded with
-        // y = 1e300, whose 2^(m - y) is 0): the lane's running minimum m of y
-        // and s = sum 2^(m - y), rescaled when a group lowers m
+        // y = +inf, whose 2^(m - y) is taken as 2^-2000 = 0): the lane's
+        // running minimum m of y and s = sum 2^(m - y), rescaled when a group
+        // lowers m
         const int qlo = (int)(((long)n0 * wave) / nw), qhi = (int)(((long)n0 * (wave + 1)) / nw);
         double m = INFINITY, s = 0.0;
+        // y of source q for this lane's target (q < qhi)
+        auto ysrc = [&](int q) {
+            const double* fr = frec + q * FR;
+            double d[D];
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(xt[ax] - fr[ax]);
+            return fma(c2, (th - fr[D]) + norm2_fma<D>(d), tT + fr[D + 1]);
+        };
         if (lane < nb) {
-            for (int q0 = qlo; q0 < qhi; q0 += UNROLL) {
-                const bool full = q0 + UNROLL <= qhi;
+            int q0 = qlo;
+            for (; q0 + UNROLL <= qhi; q0 += UNROLL) {
+                // the group's records first (all loads in flight together)
+                double rv[UNROLL][D + 2];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                    for (int f = 0; f < D + 2; ++f) rv[u][f] = frec[(q0 + u) * FR + f];
                 double y[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
-                    const double* fr = frec + (full || q0 + u < qhi ? q0 + u : q0) * FR;
                     double d[D];
 #pragma unroll
-                    for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(xt[ax] - fr[ax]);
-                    y[u] = fma(c2, (th - fr[D]) + norm2_fma<D>(d), tT + fr[D + 1]);
-                    if (!full) y[u] = q0 + u < qhi ? y[u] : 1e300;
+                    for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(xt[ax] - rv[u][ax]);
+                    y[u] = fma(c2, (th - rv[u][D]) + norm2_fma<D>(d), tT + rv[u][D + 1]);
                 }
                 double cm = y[0];
 #pragma unroll
@@ -258,11 +271,20 @@ __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay
                 s *= exp2_pair(fmax(mn - m, -2000.0));   // m = +inf on the first group: s = 0 stays 0
                 m = mn;
 #pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    double e = mn - y[u];
-                    if (!full) e = fmax(e, -2000.0);
-                    s += exp2_pair(e);
-                }
+                for (int u = 0; u < UNROLL; ++u) s += exp2_pair(mn - y[u]);
+            }
+            if (q0 < qhi) {   // the last, partial group
+                double y[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) y[u] = q0 + u < qhi ? ysrc(q0 + u) : INFINITY;
+                double cm = y[0];
+#pragma unroll
+                for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
+                const double mn = fmin(m, cm);
+                s *= exp2_pair(fmax(mn - m, -2000.0));
+                m = mn;
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) s += exp2_pair(fmax(mn - y[u], -2000.0));
             }
         }
         part[wave * REC_B + lane] = m;
@@ -455,7 +477,11 @@ __device__ __forceinline__ void xb_reduce(int N, int A, int noff, bool dforce, i
 }
 
 // Coefficients c(a,b), a <= b, of the diagonal tile of block A into dbuf
-// (0 past the block's end).  Whole workgroup.
+// (0 past the block's end), whole workgroup.  The 2080 pairs (t, t + s) are
+// taken in 33 folded offsets s = 0..32: lane t takes (t, t + s) when
+// t + s < 64 and otherwise (t + s - 64, t), the pair of offset 64 - s, so
+// that every lane has a pair (at s = 32 the upper half would repeat the
+// lower half's pairs and is idle).
 template <int D, bool PBC>
 __device__ __forceinline__ void xb_diag_coefs(int N, double c2, const double* x1, const double* Shi,
                                               const double* Tb, const double* rec, int A, double* dbuf) {
@@ -463,25 +489,21 @@ __device__ __forceinline__ void xb_diag_coefs(int N, double c2, const double* x1
     const int lane = threadIdx.x % WAVE, wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int nw = blockDim.x / WAVE;
     const int nbA = min(REC_B, N - A * REC_B);
-    const int a = A * REC_B + min(lane, nbA - 1);
-    double xa[D];
+    for (int s = wave; s <= REC_B / 2; s += nw) {
+        const bool lower = lane + s < REC_B;
+        if (s == REC_B / 2 && !lower) continue;
+        const int r = lower ? lane : lane + s - REC_B;   // row (target) and column (source) in the block
+        const int j = lower ? lane + s : lane;
+        double cf = 0.0;
+        if (j < nbA) {
+            const int a = A * REC_B + r;
+            const double* rp = rec + (A * REC_B + j) * REC;
+            double d[D];
 #pragma unroll
-    for (int ax = 0; ax < D; ++ax) xa[ax] = x1[ax * N + a];
-    const double ha = Shi[a], ta = Tb[a];
-    const int rbase = xb_row(lane);
-    for (int s = wave; s < REC_B; s += nw) {
-        const int j = lane + s;
-        if (j < REC_B) {
-            double cf = 0.0;
-            if (j < nbA) {
-                const double* rp = rec + (A * REC_B + j) * REC;
-                double d[D];
-#pragma unroll
-                for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(rp[ax] - xa[ax]);
-                cf = exp2_pair(fma(-c2, (rp[D] - ha) + norm2_fma<D>(d), rp[D + 1] + ta));
-            }
-            dbuf[rbase + s] = cf;
+            for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(rp[ax] - x1[ax * N + a]);
+            cf = exp2_pair(fma(-c2, (rp[D] - Shi[a]) + norm2_fma<D>(d), rp[D + 1] + Tb[a]));
         }
+        dbuf[xb_row(r) + (j - r)] = cf;
     }
 }
 
