@@ -124,7 +124,7 @@ __host__ __device__ inline XbLayout xb_layout(int N, int D, int nw, int nslot) {
     p = bwd;
     l.rec = p; p += N * xb_rec(D);
     l.acc = p; p += 2 * D * N;
-    l.hsum = p; p += REC_B;
+    l.hsum = p; p += 2 * REC_B;   // h, then the solve's broadcast values
     l.dbuf = p; p += XB_DIAG;
     l.slot = p; p += nslot * xb_slot(D);
     l.total = fwd_end > p ? fwd_end : p;
@@ -152,10 +152,10 @@ __device__ __forceinline__ double xb_dist2(const double* xa, int ia, const doubl
 // the chunk sums across lanes, then the running sums (the chunk's links are
 // evaluated again), stored as S_v = H_v + L_v with H_v a multiple of
 // q = 2^(e - 40), 2^e <= S_{N-1} < 2^(e+1) (as ring_prefix): H_v - H_u is exact
-// and |L_v| <= q/2.
+// and |L_v| <= q/2.  H_v also goes into the forward source records.
 template <int D, bool PBC>
 __device__ __forceinline__ void xb_prefix(const double* x1, const double* xP, int N, int lane, double* Shi,
-                                          double* Slo) {
+                                          double* Slo, double* frec) {
     const int c = (N + WAVE - 1) / WAVE;
     const int v0 = lane * c;
     DD tot{0.0, 0.0};
@@ -180,6 +180,7 @@ __device__ __forceinline__ void xb_prefix(const double* x1, const double* xP, in
             const double h = rint(run.hi * invq) * qq;
             Shi[v] = h;
             Slo[v] = (run.hi - h) + run.lo;
+            frec[v * xb_frec(D) + D] = h;
             if (v < N - 1) run = dd_add(run, DD{xb_dist2<D, PBC>(x1, v + 1, xP, v, N), 0.0});
         }
     }
@@ -441,19 +442,21 @@ __device__ __forceinline__ double xb_slot_sum(const double* slot, int sl, int of
 }
 
 // The partials of chunks [c0, cend) of round A, added per particle in chunk
-// order: h of block A into hsum, F_1 of block A (off-diagonal chunks, the
-// first noff << lcpt) and of block A+1 (the diagonal-force chunks after
-// them), F_P of the column tiles A+1 .. A+ncol (tile A+1+i's chunks, and for
-// column tile A+1 also the diagonal-force chunks).  Whole workgroup.
+// order: h of block A into hsum (part 0), F_1 of block A (off-diagonal
+// chunks, the first noff << lcpt) and of block A+1 (the diagonal-force
+// chunks after them), F_P of the column tiles A+1 .. A+ncol (tile A+1+i's
+// chunks, and for column tile A+1 also the diagonal-force chunks).  Parts
+// [p0, p1) of 1 + 2D + D ncol (p1 < 0: all), by threads t0, t0 + ts, ...
 template <int D>
 __device__ __forceinline__ void xb_reduce(int N, int A, int noff, bool dforce, int lcpt, int c0, int cend,
-                                          const double* slot, double* acc, double* hsum) {
+                                          const double* slot, double* acc, double* hsum, int p0, int p1, int t0,
+                                          int ts) {
     constexpr int SL = xb_slot(D);
     const int ncol = noff > 0 ? noff : (dforce ? 1 : 0);
-    const int nparts = 1 + 2 * D + D * ncol;
+    const int nparts = p1 < 0 ? 1 + 2 * D + D * ncol : p1;
     const int offend = min(cend, noff << lcpt);   // off-diagonal chunks end
     const int dlo = max(c0, noff << lcpt);          // diagonal-force chunks [dlo, cend)
-    for (int e = threadIdx.x; e < nparts * REC_B; e += blockDim.x) {
+    for (int e = p0 * REC_B + t0; e < nparts * REC_B; e += ts) {
         const int part = e / REC_B, l = e % REC_B;
         if (part == 0) {
             if (A >= 0) hsum[l] += xb_slot_sum(slot, SL, l, c0, c0, offend);
@@ -577,29 +580,31 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
     const double* xPm1 = x + bs * ((P - 2 + P) % P);
     double* f1 = f;
     double* fP = f + bs * (P - 1);
-    {
-        const double Lr = PBC ? L : 0.0, iLr = PBC ? 1.0 / L : 0.0;
-        for (int e = tid; e < D * N; e += nthr) {
-            const double a1 = x1g[e], aP = xPg[e];
-            const double d1 = x2[e] - a1, dP = xPm1[e] - aP;
-            const double s1 = a1 * sc;
-            x1[e] = s1;
-            frec[(e % N) * FR + e / N] = s1;
-            xP[e] = aP * sc;
-            f1[e] = fma(fma(-Lr, rint(d1 * iLr), d1), k, f1[e]);
-            fP[e] = fma(fma(-Lr, rint(dP * iLr), dP), k, fP[e]);
-        }
+    for (int e = tid; e < D * N; e += nthr) {
+        const double s1 = x1g[e] * sc;
+        x1[e] = s1;
+        frec[(e % N) * FR + e / N] = s1;
+        xP[e] = xPg[e] * sc;
     }
     for (int i = tid; i <= N; i += nthr) {
         lg2[i] = i > 0 ? log2((double)i) : 0.0;
         U[i] = 0.0;
     }
     __syncthreads();
-    if (wave == 0) xb_prefix<D, PBC>(x1, xP, N, lane, Shi, Slo);
-    __syncthreads();
-    for (int q = tid; q < N; q += nthr) {
-        frec[q * FR + D] = Shi[q];
-        if (q == 0) frec[D + 1] = 0.0;   // U_0 - c2 L_0 = 0 (S_0 = 0)
+    // the prefix sums on wave 0, the interior links on the others
+    if (wave == 0) {
+        xb_prefix<D, PBC>(x1, xP, N, lane, Shi, Slo, frec);
+        if (lane == 0) frec[D + 1] = 0.0;   // U_0 - c2 L_0 = 0 (S_0 = 0)
+    }
+    if (wave != 0 || nw == 1) {
+        const int t0 = nw == 1 ? tid : tid - WAVE, ts = nw == 1 ? nthr : nthr - WAVE;
+        const double Lr = PBC ? L : 0.0, iLr = PBC ? 1.0 / L : 0.0;
+        for (int e = t0; e < D * N; e += ts) {
+            const double a1 = x1g[e], aP = xPg[e];
+            const double d1 = x2[e] - a1, dP = xPm1[e] - aP;
+            f1[e] = fma(fma(-Lr, rint(d1 * iLr), d1), k, f1[e]);
+            fP[e] = fma(fma(-Lr, rint(dP * iLr), dP), k, fP[e]);
+        }
     }
     __syncthreads();
     // ---- forward recursion
@@ -646,34 +651,50 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
         __syncthreads();
     }
     const int T = (N + REC_B - 1) / REC_B;
+    double* out = hsum + REC_B;   // the solve's broadcast values
     for (int A = T - 1; A >= -1; --A) {
         // chunks: the off-diagonal tiles (A, B > A) and the diagonal tile of
         // block A+1 (solved in the previous round), 2^lcpt chunks per tile
         const int noff = A >= 0 ? T - 1 - A : 0;
         const bool dforce = A + 1 < T;
         const int ntiles = noff + (dforce ? 1 : 0);
-        if (ntiles > 0) {
-            int lcpt = 0;   // at most 16 chunks per tile, one chunk per wave
-            while (lcpt < 4 && (2 << lcpt) * ntiles <= nw) ++lcpt;
-            const int len = REC_B >> lcpt, nchunk = ntiles << lcpt;
-            for (int c0 = 0; c0 < nchunk; c0 += nslot) {
-                const int cend = min(nchunk, c0 + nslot);
-                const int c = c0 + wave;
-                if (c < cend) {
-                    const int tile = c >> lcpt, s0 = (c & ((1 << lcpt) - 1)) * len;
-                    double* sl = slot + (c - c0) * xb_slot(D);
-                    if (tile < noff) xb_offdiag_chunk<D, PBC>(N, c2, x1, Shi, Tb, rec, A, A + 1 + tile, s0, len, sl);
-                    else xb_diag_force_chunk<D, PBC>(N, x1, rec, dbuf, A + 1, s0, len, sl);
-                }
+        int lcpt = 0;   // at most 16 chunks per tile, one chunk per wave
+        while (lcpt < 4 && (2 << lcpt) * ntiles <= nw) ++lcpt;
+        const int len = REC_B >> lcpt, nchunk = ntiles << lcpt;
+        int c0 = 0;
+        for (; c0 < nchunk; c0 += nslot) {
+            const int cend = min(nchunk, c0 + nslot);
+            const int c = c0 + wave;
+            if (c < cend) {
+                const int tile = c >> lcpt, s0 = (c & ((1 << lcpt) - 1)) * len;
+                double* sl = slot + (c - c0) * xb_slot(D);
+                if (tile < noff) xb_offdiag_chunk<D, PBC>(N, c2, x1, Shi, Tb, rec, A, A + 1 + tile, s0, len, sl);
+                else xb_diag_force_chunk<D, PBC>(N, x1, rec, dbuf, A + 1, s0, len, sl);
+            }
+            __syncthreads();
+            if (cend < nchunk || A < 0) {   // not the round's last sub-round before a solve: everything now
+                xb_reduce<D>(N, A, noff, dforce, lcpt, c0, cend, slot, acc, hsum, 0, -1, tid, nthr);
                 __syncthreads();
-                xb_reduce<D>(N, A, noff, dforce, lcpt, c0, cend, slot, acc, hsum);
-                __syncthreads();
+            } else {
+                break;   // c0 stays at the last sub-round, reduced below
             }
         }
         if (A < 0) break;
+        // h of the last sub-round (one wave) beside the diagonal
+        // coefficients; then the solve on wave 0 while the other waves add
+        // that sub-round's F_1 and F_P partials
+        const bool pending = c0 < nchunk;
+        const int cend = min(nchunk, c0 + nslot);
+        if (pending && wave == nw - 1)
+            xb_reduce<D>(N, A, noff, dforce, lcpt, c0, cend, slot, acc, hsum, 0, 1, lane, WAVE);
         xb_diag_coefs<D, PBC>(N, c2, x1, Shi, Tb, rec, A, dbuf);
         __syncthreads();
-        if (wave == 0) xb_solve<D>(N, A, hsum, dbuf, rec, slot);   // (the slots are free here)
+        if (wave == 0) {
+            xb_solve<D>(N, A, hsum, dbuf, rec, out);
+            if (nw == 1 && pending) xb_reduce<D>(N, A, noff, dforce, lcpt, c0, cend, slot, acc, hsum, 1, -1, lane, WAVE);
+        } else if (pending) {
+            xb_reduce<D>(N, A, noff, dforce, lcpt, c0, cend, slot, acc, hsum, 1, -1, tid - WAVE, nthr - WAVE);
+        }
         __syncthreads();
         if (tid < REC_B) hsum[tid] = 0.0;
     }

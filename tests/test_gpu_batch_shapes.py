@@ -10,6 +10,8 @@
   pair walk accumulates the columns in registers) and a bosonic N=780
   replica (the exterior forces get fewer chunk slots than waves, and the
   recursions run one after the other).
+* A ring whose two-particle cycles dominate (one-particle cycles suppressed
+  by 2^-430 each), beside a weakly paired one.
 * 100 device-noise steps of the benchmark's own initial state through the
   batch against the single-system context with the same seed.
 
@@ -25,7 +27,7 @@ import pytest
 
 import oracle as O
 import pimdb_amd as G
-from helpers import aziz_config, per_bead_err, rel_err
+from helpers import AMU, KELVIN, MEV, aziz_config, per_bead_err, rel_err
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import bench  # noqa: E402  (the benchmark's workload generator)
@@ -85,6 +87,42 @@ def test_large_replicas_forces_match_oracle():
         p = np.random.default_rng(r).normal(0, 0.3, size=x.shape)
         reps.append((kw, x, p))
     errs = check_forces(reps)
+    print("per-replica force errors", errs)
+    assert max(errs) < 1e-10, errs
+
+
+def paired_ring(N=128, P=8, sep=40.0, seed=3):
+    """A ring in which every particle 2i shares a two-particle cycle with
+    2i+1 (bead P of each sits at bead 1 of the other, `sep` bohr from its own
+    bead 1).  At sep = 40 one-particle cycles cost about 430 in the base-2
+    exponent against 0.3 for the pairs: the in-block min-plus shifts span
+    thousands of units and every probability but the pairings underflows."""
+    kw = dict(natoms=N, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+              gamma=1.0 / 2000.0, bosonic=1, pbc=0, size=60.0, ext_kind="harmonic", ext_omega=0.01 * MEV,
+              int_kind="free", cutoff=-1.0, propagator="cartesian", thermostat="langevin", seed=5)
+    rng = np.random.default_rng(seed)
+    x = np.zeros((P, N, 3))
+    for i in range(N // 2):
+        A = np.array([i * 100.0, 0.0, 0.0])
+        B = A + np.array([0.0, sep, 0.0])
+        f = (np.arange(P) / (P - 1))[:, None]
+        x[:, 2 * i] = A + f * (B - A)
+        x[:, 2 * i + 1] = B + f * (A - B)
+    x += rng.normal(0, 0.3, x.shape)
+    x -= x.mean(axis=(0, 1))
+    p = rng.normal(0, 0.3, size=x.shape)
+    return kw, x, p
+
+
+def test_paired_rings_forces_match_oracle():
+    """Forces of the strongly paired ring (sep 40 bohr) and of a weakly paired
+    one (N=100, sep 1 bohr) against the oracle per bead."""
+    kw, x, p = paired_ring()
+    d = O.derived(O.make_params(**kw))
+    o = O.exchange(x[0], x[1], x[-2], x[-1], d["beta_t"], d["k"], False, 0.0)
+    assert o["prob"][1, 0] > 0.999   # particle 0 closes its cycle through particle 1
+    kw2, x2, p2 = paired_ring(N=100, sep=1.0, seed=4)
+    errs = check_forces([(kw, x, p), (kw2, x2, p2)])
     print("per-replica force errors", errs)
     assert max(errs) < 1e-10, errs
 
