@@ -99,7 +99,7 @@ struct XbLayout {
     int x1, Shi, Slo;                 // bead 1 [D][N], the split sums H [N], L [N]
     int U2, xP, lg2, frec;            // U [N+1], bead P [D][N], log2 n [N+1], source records [N]
     int R, z, part, psum, tri;        // forward: shifts, substitution results, partials, in-block matrix
-    int rec, acc, hsum, dbuf, slot;   // backward: records [N], sums [2][D][N], h [64], diagonal tile, chunk slots
+    int rec, acc, hsum, dbuf, slot;   // backward: records [N], sums (F_1 [2][D][64], F_P [D][N]), h [64] + [64], diagonal tile, chunk slots
     int total;
 };
 
@@ -123,7 +123,7 @@ __host__ __device__ inline XbLayout xb_layout(int N, int D, int nw, int nslot) {
     const int fwd_end = p;
     p = bwd;
     l.rec = p; p += N * xb_rec(D);
-    l.acc = p; p += 2 * D * N;
+    l.acc = p; p += 2 * D * REC_B + D * N;
     l.hsum = p; p += 2 * REC_B;   // h, then the solve's broadcast values
     l.dbuf = p; p += XB_DIAG;
     l.slot = p; p += nslot * xb_slot(D);
@@ -447,6 +447,8 @@ __device__ __forceinline__ double xb_slot_sum(const double* slot, int sl, int of
 // chunks after them), F_P of the column tiles A+1 .. A+ncol (tile A+1+i's
 // chunks, and for column tile A+1 also the diagonal-force chunks).  Parts
 // [p0, p1) of 1 + 2D + D ncol (p1 < 0: all), by threads t0, t0 + ts, ...
+// acc: F_1 of a row block at [(block & 1) D + axis][64] (a row block's sums
+// are final one round after they start), then F_P at [axis][N].
 template <int D>
 __device__ __forceinline__ void xb_reduce(int N, int A, int noff, bool dforce, int lcpt, int c0, int cend,
                                           const double* slot, double* acc, double* hsum, int p0, int p1, int t0,
@@ -462,10 +464,10 @@ __device__ __forceinline__ void xb_reduce(int N, int A, int noff, bool dforce, i
             if (A >= 0) hsum[l] += xb_slot_sum(slot, SL, l, c0, c0, offend);
         } else if (part <= D) {
             const int ax = part - 1, row = A * REC_B + l;
-            if (A >= 0 && row < N) acc[ax * N + row] += xb_slot_sum(slot, SL, (1 + ax) * REC_B + l, c0, c0, offend);
+            if (A >= 0 && row < N) acc[((A & 1) * D + ax) * REC_B + l] += xb_slot_sum(slot, SL, (1 + ax) * REC_B + l, c0, c0, offend);
         } else if (part <= 2 * D) {
             const int ax = part - 1 - D, row = (A + 1) * REC_B + l;
-            if (dforce && row < N) acc[ax * N + row] += xb_slot_sum(slot, SL, (1 + ax) * REC_B + l, c0, dlo, cend);
+            if (dforce && row < N) acc[(((A + 1) & 1) * D + ax) * REC_B + l] += xb_slot_sum(slot, SL, (1 + ax) * REC_B + l, c0, dlo, cend);
         } else {
             const int q = part - 1 - 2 * D, i = q / D, ax = q % D, col = (A + 1 + i) * REC_B + l;
             if (col < N) {
@@ -473,7 +475,7 @@ __device__ __forceinline__ void xb_reduce(int N, int A, int noff, bool dforce, i
                 double sum = 0.0;
                 if (i < noff) sum = xb_slot_sum(slot, SL, off, c0, max(c0, i << lcpt), min(cend, (i + 1) << lcpt));
                 if (i == 0) sum += xb_slot_sum(slot, SL, off, c0, dlo, cend);
-                acc[(D + ax) * N + col] += sum;
+                acc[2 * D * REC_B + ax * N + col] += sum;
             }
         }
     }
@@ -646,12 +648,44 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
                 Tb[b] = rv[r][D + 1];
             }
         }
-        for (int e = tid; e < 2 * D * N; e += nthr) acc[e] = 0.0;
+        for (int e = tid; e < 2 * D * REC_B + D * N; e += nthr) acc[e] = 0.0;
         if (tid < REC_B) hsum[tid] = 0.0;
         __syncthreads();
     }
     const int T = (N + REC_B - 1) / REC_B;
     double* out = hsum + REC_B;   // the solve's broadcast values
+    const double kL = k * (PBC ? L : 1.0);
+    // F_1 of row block B is final: with the links b = a - 1 (P[l-1][l] =
+    // 1 - G_l, record l-1) into f, f_1 += k L (S + links) (the interior links
+    // went in at the start), and its sums cleared for block B - 2
+    auto flush_rows = [&](int B) {
+        for (int e = tid; e < REC_B; e += nthr) {
+            const int l = B * REC_B + e;
+            double* s1 = acc + (B & 1) * D * REC_B + e;
+            if (l < N) {
+                double x1l[D], v[D];
+#pragma unroll
+                for (int ax = 0; ax < D; ++ax) {
+                    x1l[ax] = x1[ax * N + l];
+                    v[ax] = s1[ax * REC_B];
+                }
+                if (l >= 1) {
+                    const double* rp = rec + (l - 1) * REC;
+                    const double p1 = 1.0 - rp[D + 2];
+                    if (isnan(p1)) atomicOr(status, 1);
+#pragma unroll
+                    for (int ax = 0; ax < D; ++ax) v[ax] = fma(p1, xb_image<PBC>(rp[ax] - x1l[ax]), v[ax]);
+                }
+#pragma unroll
+                for (int ax = 0; ax < D; ++ax) {
+                    const size_t i = (size_t)ax * N + l;
+                    f1[i] = fma(v[ax], kL, f1[i]);
+                }
+            }
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) s1[ax * REC_B] = 0.0;
+        }
+    };
     for (int A = T - 1; A >= -1; --A) {
         // chunks: the off-diagonal tiles (A, B > A) and the diagonal tile of
         // block A+1 (solved in the previous round), 2^lcpt chunks per tile
@@ -679,7 +713,10 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
                 break;   // c0 stays at the last sub-round, reduced below
             }
         }
-        if (A < 0) break;
+        if (A < 0) {
+            flush_rows(0);
+            break;
+        }
         // h of the last sub-round (one wave) beside the diagonal
         // coefficients; then the solve on wave 0 while the other waves add
         // that sub-round's F_1 and F_P partials
@@ -697,26 +734,15 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
         }
         __syncthreads();
         if (tid < REC_B) hsum[tid] = 0.0;
+        if (A + 1 < T) flush_rows(A + 1);   // (its diagonal forces were added this round)
     }
-    // ---- the links b = a - 1 and the sums into f (k L (S + links); the
-    // interior links went in at the start)
-    const double kL = k * (PBC ? L : 1.0);
+    // ---- bead P: the links P[l][l+1] = 1 - G_{l+1} (record l) and the sums
+    // into f, f_P += k L (S + links)
     for (int l = tid; l < N; l += nthr) {
-        double s1[D], sP[D], x1l[D];
+        double sP[D];
 #pragma unroll
-        for (int ax = 0; ax < D; ++ax) {
-            x1l[ax] = x1[ax * N + l];
-            s1[ax] = acc[ax * N + l];
-            sP[ax] = acc[(D + ax) * N + l];
-        }
-        if (l >= 1) {   // P[l-1][l] = 1 - G_l (record l-1): bead 1 of l from bead P of l-1
-            const double* rp = rec + (l - 1) * REC;
-            const double p1 = 1.0 - rp[D + 2];
-            if (isnan(p1)) atomicOr(status, 1);
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) s1[ax] = fma(p1, xb_image<PBC>(rp[ax] - x1l[ax]), s1[ax]);
-        }
-        if (l + 1 < N) {   // P[l][l+1] = 1 - G_{l+1} (record l): bead P of l to bead 1 of l+1
+        for (int ax = 0; ax < D; ++ax) sP[ax] = acc[2 * D * REC_B + ax * N + l];
+        if (l + 1 < N) {
             const double* rp = rec + l * REC;
             const double pP = 1.0 - rp[D + 2];
 #pragma unroll
@@ -725,7 +751,6 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
 #pragma unroll
         for (int ax = 0; ax < D; ++ax) {
             const size_t i = (size_t)ax * N + l;
-            f1[i] = fma(s1[ax], kL, f1[i]);
             fP[i] = fma(sP[ax], kL, fP[i]);
         }
     }
