@@ -587,9 +587,13 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
 // Physical forces and interior springs on every (bead, particle) of a replica
 // (simulation.cpp:353-455).  Each wave owns whole beads: the pairs of a bead
 // with Newton's third law (slice_pair_forces), then the trap and the springs.
+// after_exch_: the exchange phase has already put its forces into f of beads
+// 1 and P (only with the pair forces in LDS), which then get the physical
+// forces added instead of written.
 template <int D, int KIND>
-__device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp, double* lds_) {
+__device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp, double* lds_, int after_exch_) {
     const RepDesc r = rep_uniform(r_);
+    const bool after_exch = uni(after_exch_) != 0;
     lds_double* F = (lds_double*)uni(lds_);
     const int N = r.N, P = r.P;
     const double* x = uni(bp.x) + r.off_x;
@@ -670,7 +674,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
                     const double dn = disp(xv[a], xnext[e], pbc, r.L);
                     spring = r.k * (dp + dn);
                 }
-                fo[e] = exchange_bead ? phys : spring + phys;
+                fo[e] = exchange_bead ? (after_exch ? fo[e] + phys : phys) : spring + phys;
             }
         }
     }
@@ -685,12 +689,13 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
 
 // Exchange of one replica (exchange_batch.hpp): the forward recursion, then
 // the backward recursion in linear form with the exterior forces added into f
-// of beads 1 and P.  Kept out of line (its registers would otherwise add to
+// of beads 1 and P (add_ = 0: written into them).  Kept out of line (its registers would otherwise add to
 // every other phase's); the arguments are made wave-uniform on entry.
 template <int D>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
-                                         double* f_, double* V_, int* status_, double* lds_, int slots_) {
+                                         double* f_, double* V_, int* status_, double* lds_, int slots_, int add_) {
     const int N = uni(N_), P = uni(P_);
+    const bool add = uni(add_) != 0;
     const int slots = uni(slots_);
     const double L = uni(L_), k = uni(k_), bt = uni(bt_);
     const double* x = uni(x_);
@@ -698,8 +703,8 @@ __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, do
     double* V = uni(V_);
     int* status = uni(status_);
     double* lds = uni(lds_);
-    if (uni(pbc_) != 0) xb_exchange<D, true>(N, P, L, k, bt, x, f, V, status, lds, slots);
-    else xb_exchange<D, false>(N, P, L, k, bt, x, f, V, status, lds, slots);
+    if (uni(pbc_) != 0) xb_exchange<D, true>(N, P, L, k, bt, x, f, V, status, lds, slots, add);
+    else xb_exchange<D, false>(N, P, L, k, bt, x, f, V, status, lds, slots, add);
 }
 
 // Work of one launch: a full MD step, or parts of it for timing.
@@ -725,10 +730,19 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
             wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
             if (!r.fused) wg_kick_drift<D>(r, bp, cm);
         }
-        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds);
-        if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic)
+        // the second half of the grid (with two workgroups per compute unit,
+        // mostly the other workgroup of each unit) takes the exchange before
+        // the pairs, so that one workgroup's exchange meets the other's pair
+        // walk instead of its own exchange
+        const bool xfirst = MODE <= BM_FORCES && r.bosonic && r.plds && KIND != POT_FREE && r.cutoff != 0.0 &&
+                            2 * (int)blockIdx.x >= (int)gridDim.x;   // (pair forces in LDS: f written once)
+        if (xfirst)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.status + ri, lds, r.xslots);
+                           bp.status + ri, lds, r.xslots, 0);
+        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds, xfirst ? 1 : 0);
+        if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic && !xfirst)
+            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                           bp.status + ri, lds, r.xslots, 1);
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];

@@ -550,13 +550,14 @@ __device__ __forceinline__ void xb_solve(int N, int A, const double* hsum, const
 }
 
 // The whole exchange of one ring on the workgroup: V (the reference's units)
-// and the exterior spring forces added into beads 1 and P of f
-// (f = F_exchange + f_phys, simulation.cpp:363-373).  x, f: bead-major SoA
+// and the exterior spring forces added into beads 1 and P of f (f =
+// F_exchange + f_phys, simulation.cpp:363-373), or written there (add false)
+// for a caller that adds the physical forces afterwards.  x, f: bead-major SoA
 // [P][D][N].  nslot: chunk slots of the layout (1 <= nslot <= waves).  Every
 // thread calls it.
 template <int D, bool PBC>
 __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, double bt, const double* x, double* f,
-                                            double* V, int* status, double* lds, int nslot) {
+                                            double* V, int* status, double* lds, int nslot, bool add) {
     constexpr int REC = xb_rec(D);
     constexpr int FR = xb_frec(D);
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -604,8 +605,8 @@ __device__ __forceinline__ void xb_exchange(int N, int P, double L, double k, do
         for (int e = t0; e < D * N; e += ts) {
             const double a1 = x1g[e], aP = xPg[e];
             const double d1 = x2[e] - a1, dP = xPm1[e] - aP;
-            f1[e] = fma(fma(-Lr, rint(d1 * iLr), d1), k, f1[e]);
-            fP[e] = fma(fma(-Lr, rint(dP * iLr), dP), k, fP[e]);
+            f1[e] = fma(fma(-Lr, rint(d1 * iLr), d1), k, add ? f1[e] : 0.0);
+            fP[e] = fma(fma(-Lr, rint(dP * iLr), dP), k, add || P == 1 ? fP[e] : 0.0);   // (P = 1: the same bead)
         }
     }
     __syncthreads();
