@@ -237,7 +237,7 @@ def traffic_from_profile():
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same
     counter's reading on a copy kernel of known size), or None."""
-    path = os.path.join(REPO, "profiles", "k_batch_step_aziz_n256_p64_r256_traffic.txt")
+    path = os.path.join(REPO, "profiles", "k_batch_step_aziz_n256_p64_r512_traffic.txt")
     if not os.path.exists(path):
         return None
     vals = {}
