@@ -607,8 +607,12 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     FracParams fp = make_frac_params(pp);
     fp.cut2 = uni(fp.cut2); fp.kh = uni(fp.kh); fp.kd = uni(fp.kd); fp.ex = uni(fp.ex);
     fp.b0 = uni(fp.b0); fp.b1 = uni(fp.b1); fp.b2 = uni(fp.b2); fp.dl = uni(fp.dl);
-    lds_double* Fw = F + (size_t)wave * bs;   // this wave's slice forces (r.plds)
-    lds_int* perm = (lds_int*)(F + (size_t)nw * bs);   // [N] tile order of the particles (r.plds)
+    // this wave's slice forces by tile position (r.plds), then the tile
+    // order of the particles and its inverse
+    const size_t fs = (size_t)pair_fstride(D) * N;
+    lds_double* Fw = F + (size_t)wave * fs;
+    lds_int* perm = (lds_int*)(F + (size_t)nw * fs);   // [N] particle at each tile position
+    lds_int* iperm = perm + N;                          // [N] tile position of each particle
     if (KIND != POT_FREE && r.cutoff != 0.0 && r.plds) {
         // Aziz with PBC: a spatial order from bead 0 (pair_order_key), which
         // every bead shares; otherwise the identity.  The keys sit in the
@@ -628,6 +632,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
             int rank = 0;
             for (int j = 0; j < N; ++j) rank += key[j] < ki ? 1 : 0;
             perm[rank] = i;
+            iperm[i] = rank;
         }
         __syncthreads();
     }
@@ -661,13 +666,14 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
         const double* xnext = x + bs * ((b + 1) % P);
         for (int i = lane; i < N; i += WAVE) {
             double xv[D], fe[D];
+            const int ip = in_lds ? iperm[i] * pair_fstride(D) : 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) xv[a] = xs[a * N + i];
             ext_force<D>(r.ext, xv, fe);
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 const int e = a * N + i;
-                const double phys = fe[a] + (in_lds ? Fw[i * D + a] : fo[e]);
+                const double phys = fe[a] + (in_lds ? Fw[ip + a] : fo[e]);
                 double spring = 0.0;
                 if (!exchange_bead) {
                     const double dp = disp(xv[a], xprev[e], pbc, r.L);

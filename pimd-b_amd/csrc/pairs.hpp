@@ -306,81 +306,100 @@ struct WalkGrad {
     }
 };
 
-// All pairs of one slice on one wave, with the column forces accumulated in
-// the wave's own LDS array F[N][D] (zeroed here; the pair forces are left in
-// it: the caller adds the rest of the force and writes the slice once).  Used
-// when every wave's slice fits the LDS.
+// Doubles per tile position in the LDS force arrays of the slice walk: D
+// rounded up to odd, so that 32 consecutive positions (64-bit LDS accesses
+// of half a wave) cover 32 distinct bank pairs.
+__host__ __device__ constexpr int pair_fstride(int D) { return D | 1; }
+
+// All pairs of one slice on one wave, with the forces accumulated in the
+// wave's own LDS array F (zeroed here; the pair forces are left in it: the
+// caller adds the rest of the force and writes the slice once).  Used when
+// every wave's slice fits the LDS.
 //
 // The tiles are cut from the particles in the order perm[0..N) (a
-// permutation of 0..N-1, in LDS): tile position k holds particle perm[k],
-// and positions and forces are addressed by the particle itself, so any
-// permutation gives the same set of pairs and the forces in particle order.
-// The caller orders the particles spatially (pair_order_key) so that the
-// close pairs, where a branch of the pair force is taken, meet at few
-// offsets of the rotation.
+// permutation of 0..N-1, in LDS): tile position k holds particle perm[k].
+// Positions are read by the particle itself, so any permutation gives the
+// same set of pairs; forces are accumulated by tile position (F[k S + a],
+// S = pair_fstride(D)), so the 64 lanes of a tile update 64 consecutive
+// positions, and with S odd any 32 consecutive lanes fall on distinct LDS
+// bank pairs, for every rotation offset.  The caller reads particle i's force
+// at its position iperm[i], and orders the particles spatially
+// (pair_order_key) so that the close pairs, where a branch of the pair force
+// is taken, meet at few offsets of the rotation.
 //
-// Row tiles are taken two at a time where they can be: for tile pairs (I, J)
-// and (I+1, J) with J >= I+2, lane l holds rows 64I+l and 64(I+1)+l and one
-// rotating column of tile J, so the column's DPP rotation and its LDS update
-// (the two pairs' terms added first) serve two pairs.  The tiles (I, I),
-// (I, I+1), (I+1, I+1) and an odd last row tile take one row at a time.
+// Every pass holds two rows per lane against one rotating column, so that the
+// column's DPP rotation and its LDS update (the two pairs' terms added first)
+// serve two pairs:
+//  - a diagonal tile I: lane l holds rows l and l+32 (mod 64) of the tile and
+//    the column l+s.  Offsets s = 1..15 give the pairs at distance s (first
+//    row) and 32-s (second row) along the tile's ring of 64 positions; s = 16
+//    the distance 16 once (first row), and s = 0 the distance 32 (second row,
+//    lanes < 32), each pair of the tile exactly once;
+//  - off-diagonal tile pairs, two row tiles against one column tile: column
+//    J >= 1 takes rows 0..J-1 for J even and rows 1..J-1 for J odd, in pairs
+//    of consecutive tiles, and column 0 the odd row tiles (1,3), (5,7), ...
+//    (the pairs {0, J}, J odd, oriented the other way), so that at most one
+//    pass (column 0's last, when the number of odd tiles is odd) has a single
+//    row.
 template <int D, int KIND, int IMG>
-__device__ __forceinline__ void pair_tile_rows(const gbl_cdptr xs, int N, const WalkGrad<D, KIND, IMG>& wg, int lane,
-                                               int J, bool diag, const double (&xi)[D], bool row_ok,
-                                               double (&racc)[D], lds_double* F, const lds_int* perm) {
-    const int s0 = diag ? 1 : 0, s1 = diag ? 33 : PTILE;
-    const int kc = J * PTILE + ((lane + s0) & (PTILE - 1));
-    const int jc0 = kc < N ? perm[kc] : N;
-    // the column's index, as its first element in F (jd = jc D), moves with it
-    unsigned jd = (unsigned)(jc0 * D);
-    double xj[D];
+struct SliceWalk {
+    gbl_cdptr xs;
+    int N;
+    const WalkGrad<D, KIND, IMG>& wg;
+    lds_double* F;
+    const lds_int* perm;
+    int lane;
+    double xj[D];   // the lane's column: positions,
+    unsigned jd;    // and its first element in F (kc S); N S or above: none
+
+    __device__ __forceinline__ bool load_row(int k, double (&xr)[D]) const {
+        const bool ok = k < N;
+        const int i = ok ? perm[k] : 0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? wg.load(xs, a * N + jc0) : 0.0;
-    // the pair of this lane at the current offset, if `act`
-    auto step = [&](bool act) {
+        for (int a = 0; a < D; ++a) xr[a] = ok ? wg.load(xs, a * N + i) : 0.0;
+        return ok;
+    }
+    __device__ __forceinline__ void flush_row(int k, bool ok, const double (&r)[D]) const {
+        if (ok) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                __hip_atomic_fetch_add(&F[k * pair_fstride(D) + a], r[a], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __device__ __forceinline__ void load_col(int kc) {
+        const int jc = kc < N ? perm[kc] : N;
+        jd = (unsigned)(kc * pair_fstride(D));
+#pragma unroll
+        for (int a = 0; a < D; ++a) xj[a] = jc < N ? wg.load(xs, a * N + jc) : 0.0;
+    }
+    __device__ __forceinline__ bool col_ok() const { return jd < (unsigned)(N * pair_fstride(D)); }
+    __device__ __forceinline__ void rotate() {
+#pragma unroll
+        for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
+        jd = (unsigned)lane_rotate((int)jd);
+    }
+    // the pair (row xi, column) if act
+    __device__ __forceinline__ void one(const double (&xi)[D], double (&ri)[D], bool act) {
         if (act) {
             double d[D], g;
             if (wg(xi, xj, d, g)) {
                 lds_double* Fj = F + jd;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    racc[a] = fma(-g, d[a], racc[a]);
+                    ri[a] = fma(-g, d[a], ri[a]);
                     __hip_atomic_fetch_add(&Fj[a], g * d[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
-#pragma unroll
-        for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
-        jd = (unsigned)lane_rotate((int)jd);
-    };
-    if ((J + 1) * PTILE <= N) {
-        // full row and column tiles (J >= the row tile): every lane's pair
-        // exists, except at a diagonal tile's offset 32 (lanes < 32 only)
-        const int s2 = diag ? 32 : PTILE;
-        for (int s = s0; s < s2; ++s) step(true);
-        if (diag) step(lane < 32);
-    } else {
-        for (int s = s0; s < s1; ++s) step(row_ok && jd < (unsigned)(N * D) && (!diag || s < 32 || lane < 32));
     }
-}
-
-// Rows 64I+l and 64(I+1)+l (both present) against the full column tile J.
-template <int D, int KIND, int IMG>
-__device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, const WalkGrad<D, KIND, IMG>& wg,
-                                                   int lane, int J, const double (&xa)[D], const double (&xb)[D],
-                                                   double (&ra)[D], double (&rb)[D], lds_double* F,
-                                                   const lds_int* perm) {
-    const int kc = J * PTILE + lane;
-    const int jc0 = kc < N ? perm[kc] : N;
-    unsigned jd = (unsigned)(jc0 * D);
-    double xj[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) xj[a] = jc0 < N ? wg.load(xs, a * N + jc0) : 0.0;
-    auto step = [&](bool act) {
-        if (act) {
+    // the pairs (row xa, column) if acta and (row xb, column) if actb
+    __device__ __forceinline__ void two(const double (&xa)[D], const double (&xb)[D], double (&ra)[D],
+                                        double (&rb)[D], bool acta, bool actb) {
+        if (acta || actb) {
             double da[D], db[D], ga = 0.0, gb = 0.0;
-            const bool ina = wg(xa, xj, da, ga);
-            const bool inb = wg(xb, xj, db, gb);
+            const bool ina = wg(xa, xj, da, ga) && acta;
+            const bool inb = wg(xb, xj, db, gb) && actb;
             if (ina || inb) {
                 if (!ina) ga = 0.0;
                 if (!inb) gb = 0.0;
@@ -394,56 +413,73 @@ __device__ __forceinline__ void pair_tile_two_rows(const gbl_cdptr xs, int N, co
                 }
             }
         }
-#pragma unroll
-        for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
-        jd = (unsigned)lane_rotate((int)jd);
-    };
-    if ((J + 1) * PTILE <= N) {   // a full column tile: every lane's column exists
-        for (int s = 0; s < PTILE; ++s) step(true);
-    } else {
-        for (int s = 0; s < PTILE; ++s) step(jd < (unsigned)(N * D));
     }
-}
+
+    // the pairs of diagonal tile I
+    __device__ __forceinline__ void diag_tile(int I) {
+        const int ka = I * PTILE + lane, kb = I * PTILE + ((lane + PTILE / 2) & (PTILE - 1));
+        double xa[D], xb[D], ra[D] = {}, rb[D] = {};
+        const bool oka = load_row(ka, xa), okb = load_row(kb, xb);
+        load_col(ka);
+        one(xb, rb, okb && lane < PTILE / 2 && col_ok());   // s = 0: distance 32
+        rotate();
+        if ((I + 1) * PTILE <= N) {
+            for (int s = 1; s < PTILE / 4; ++s) {
+                two(xa, xb, ra, rb, true, true);
+                rotate();
+            }
+            one(xa, ra, true);   // s = 16
+        } else {
+            for (int s = 1; s < PTILE / 4; ++s) {
+                const bool c = col_ok();
+                two(xa, xb, ra, rb, oka && c, okb && c);
+                rotate();
+            }
+            one(xa, ra, oka && col_ok());
+        }
+        flush_row(ka, oka, ra);
+        flush_row(kb, okb, rb);
+    }
+    // row tiles A and B (B < 0: none) against column tile J, all distinct
+    __device__ __forceinline__ void off_tiles(int A, int B, int J) {
+        const int ka = A * PTILE + lane, kb = B * PTILE + lane;
+        double xa[D], xb[D], ra[D] = {}, rb[D] = {};
+        const bool oka = load_row(ka, xa), okb = B >= 0 && load_row(kb, xb);
+        load_col(J * PTILE + lane);
+        const bool full = (A + 1) * PTILE <= N && (J + 1) * PTILE <= N && (B < 0 || (B + 1) * PTILE <= N);
+        if (B < 0) {
+            for (int s = 0; s < PTILE; ++s) {
+                one(xa, ra, oka && col_ok());
+                rotate();
+            }
+        } else if (full) {
+            for (int s = 0; s < PTILE; ++s) {
+                two(xa, xb, ra, rb, true, true);
+                rotate();
+            }
+        } else {
+            for (int s = 0; s < PTILE; ++s) {
+                const bool c = col_ok();
+                two(xa, xb, ra, rb, oka && c, okb && c);
+                rotate();
+            }
+        }
+        flush_row(ka, oka, ra);
+        if (B >= 0) flush_row(kb, okb, rb);
+    }
+};
 
 template <int D, int KIND, int IMG>
 __device__ __forceinline__ void slice_pair_forces_lds(const double* xs_, int N, const WalkGrad<D, KIND, IMG>& wg,
                                                       int lane, lds_double* F, const lds_int* perm) {
-    const gbl_cdptr xs = (gbl_cdptr)xs_;
-    for (int e = lane; e < D * N; e += WAVE) F[e] = 0.0;
+    for (int e = lane; e < pair_fstride(D) * N; e += WAVE) F[e] = 0.0;
     __builtin_amdgcn_wave_barrier();
+    SliceWalk<D, KIND, IMG> w{(gbl_cdptr)xs_, N, wg, F, perm, lane, {}, 0u};
     const int T = (N + PTILE - 1) / PTILE;
-    for (int I = 0; I < T; I += 2) {
-        const bool two = I + 1 < T;
-        const int ka = I * PTILE + lane, kb = (I + 1) * PTILE + lane;
-        const bool oka = ka < N, okb = two && kb < N;
-        const int ia = oka ? perm[ka] : N, ib = okb ? perm[kb] : N;
-        double xa[D], xb[D], ra[D], rb[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            xa[a] = oka ? wg.load(xs, a * N + ia) : 0.0;
-            xb[a] = okb ? wg.load(xs, a * N + ib) : 0.0;
-            ra[a] = 0.0;
-            rb[a] = 0.0;
-        }
-        pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I, true, xa, oka, ra, F, perm);
-        if (two) {
-            pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I + 1, false, xa, oka, ra, F, perm);
-            pair_tile_rows<D, KIND, IMG>(xs, N, wg, lane, I + 1, true, xb, okb, rb, F, perm);
-            // a tile J >= I+2 exists only when tiles I and I+1 are full
-            for (int J = I + 2; J < T; ++J)
-                pair_tile_two_rows<D, KIND, IMG>(xs, N, wg, lane, J, xa, xb, ra, rb, F, perm);
-        }
-        if (oka) {
-#pragma unroll
-            for (int a = 0; a < D; ++a)
-                __hip_atomic_fetch_add(&F[ia * D + a], ra[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-        if (okb) {
-#pragma unroll
-            for (int a = 0; a < D; ++a)
-                __hip_atomic_fetch_add(&F[ib * D + a], rb[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
+    for (int I = 0; I < T; ++I) w.diag_tile(I);
+    for (int J = 1; J < T; ++J)
+        for (int A = J & 1; A < J; A += 2) w.off_tiles(A, A + 1, J);
+    for (int A = 1; A < T; A += 4) w.off_tiles(A, A + 2 < T ? A + 2 : -1, 0);
     __builtin_amdgcn_wave_barrier();
 }
 

@@ -119,8 +119,9 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
             return 0;
         }
         need = std::max(need, nm);
-        // pair forces of every wave's slice in LDS when they fit (+ the tile order)
-        const size_t pl = (size_t)nw * D * N + (N + 1) / 2;
+        // pair forces of every wave's slice in LDS when they fit (+ the tile
+        // order and its inverse, N ints each)
+        const size_t pl = (size_t)nw * pair_fstride(D) * N + N;
         d.plds = pl <= avail ? 1 : 0;
         if (d.plds) need = std::max(need, pl);
     }
