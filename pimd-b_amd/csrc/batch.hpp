@@ -607,6 +607,9 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     FracParams fp = make_frac_params(pp);
     fp.cut2 = uni(fp.cut2); fp.kh = uni(fp.kh); fp.kd = uni(fp.kd); fp.ex = uni(fp.ex);
     fp.b0 = uni(fp.b0); fp.b1 = uni(fp.b1); fp.b2 = uni(fp.b2); fp.dl = uni(fp.dl);
+    // b2 and b1 meet in one FMA of the pair loop: copies in vector registers
+    fp.b1v = vgpr_copy(fp.b1);
+    fp.b2v = vgpr_copy(fp.b2);
     // this wave's slice forces by tile position (r.plds), then the tile
     // order of the particles and its inverse
     const size_t fs = (size_t)pair_fstride(D) * N;

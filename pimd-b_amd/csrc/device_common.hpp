@@ -89,6 +89,16 @@ __device__ __forceinline__ double norm2_fma(const double (&d)[D]) {
     return s;
 }
 
+// A copy of a wave-uniform double in vector registers, for an operand that
+// meets another scalar one in the same instruction (one scalar operand per
+// VALU instruction): the compiler would otherwise copy it to a VGPR at every
+// use, and under scalar register pressure reload it from a spill first.
+__device__ __forceinline__ double vgpr_copy(double s) {
+    double v;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+
 // a * b + c with c in a scalar register.  The compiler turns p = fma(p, r, c)
 // with a loop-invariant c into a copy of c plus v_fmac; a three-operand
 // v_fma_f64 with the constant in an SGPR needs neither.

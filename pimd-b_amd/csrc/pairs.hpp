@@ -130,6 +130,7 @@ struct FracParams {
     double kh, kd;          // harmonic and dipole factors
     double ex;              // -alpha L / rm * log2(e): the repulsion's exponent per unit r_s
     double b0, b1, b2;      // undamped dispersion
+    double b1v, b2v;        // b1 and b2 again, for copies in vector registers (vgpr_copy)
     double dl;              // D rm / L: the damping range in r_s, and t = dl y - 1
 };
 
@@ -144,7 +145,9 @@ __host__ __device__ inline FracParams make_frac_params(const PairParams& pp) {
     f.ex = -Aziz::alpha * (L / Aziz::rm) * 1.4426950408889634;
     f.b0 = 6.0 * Aziz::C6 * ER * a7;
     f.b1 = 8.0 * Aziz::C8 * ER * a7 * a2;
+    f.b1v = f.b1;
     f.b2 = 10.0 * Aziz::C10 * ER * a7 * a2 * a2;
+    f.b2v = f.b2;
     f.dl = Aziz::D * a;
     return f;
 }
@@ -198,7 +201,7 @@ __device__ __forceinline__ bool pair_grad_frac(const double (&xi)[D], const doub
         const double e1 = fp.ex * rs, k1 = rint(e1);
         const double T1 = ldexp(frac_exp2(e1 - k1), (int)k1);
         const double q = y * y, q2 = q * q, q4 = q2 * q2;
-        double att = q4 * fma(fma(fp.b2, q, fp.b1), q, fp.b0);
+        double att = q4 * fma(fma(fp.b2v, q, fp.b1v), q, fp.b0);
         if (rs < fp.dl) {
             const double t = fma(fp.dl, y, -1.0);
             const double e2 = (t * -1.4426950408889634) * t, k2 = rint(e2);
