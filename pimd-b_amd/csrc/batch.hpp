@@ -33,6 +33,9 @@ struct RepDesc {
     unsigned long long off_x;   // P*D*N block of x, p, f
     unsigned long long off_V;   // N+1 block of V
     unsigned long long off_C;   // P*P block of Ct, CTt
+    unsigned long long off_R;   // P*4 block of rot (nmprop)
+    int nmprop;                 // normal-mode propagator (distinguishable particles): the kicks take the
+                                // physical forces fx, the drift is wg_nm_drift's
 };
 
 struct BatchPtrs {
@@ -44,6 +47,8 @@ struct BatchPtrs {
     int* status;               // per replica: 1 when the exchange overflowed
     unsigned long long* ctl;   // [0] step counter, [1] finished-workgroup counter
     double* cmp;               // per replica [8]: pending COM momentum [0..3], known sum of p [4..7]
+    double* fx;                // physical forces (no springs) of nmprop replicas; f when there are none
+    const double* rot;         // per replica [P][4]: cos(w dt), sin(w dt), m w, w of each mode (nmprop)
 };
 
 // Sum over the workgroup of one double per thread; result in every thread.
@@ -77,7 +82,9 @@ __device__ __forceinline__ T* uni(T* p) {
 // Wave-uniform copy of a replica descriptor.  The phases below are kept out
 // of line (each one's registers would otherwise add to the others'); their
 // arguments arrive in vector registers, and these copies put the replica's
-// sizes, constants and offsets back into scalar registers.
+// sizes, constants and offsets back into scalar registers.  (nmprop and
+// off_R are read where they are used: the copy is kept as the other phases
+// need it.)
 __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
@@ -170,7 +177,9 @@ __device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double
 // S the sum of p before the step (cmp[4..]: 0 after a deferred subtraction,
 // the sum of the state a caller set).  p then reads back through
 // pimdb_batch_get_state with the pending COM removed.
-template <int D>
+// NMP: the kernel of a batch with normal-mode-propagator replicas (their
+// kicks take fx).
+template <int D, bool NMP = false>
 __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, bool kick_, uint32_t step_,
                                         uint32_t ostep_, double* red_, double* lds_, bool subtract_, double (&cm)[D],
                                         double* cmp_) {
@@ -183,7 +192,8 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     double* lds = uni(lds_);
     const int N = r.N, P = r.P;
     double* p = uni(bp.p) + r.off_x;
-    const double* f = uni(bp.f) + r.off_x;
+    // the kick's forces: the physical ones alone with the normal-mode propagator
+    const double* f = uni((NMP && uni(r_.nmprop)) ? (const double*)bp.fx : (const double*)bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
     const int ncols = D * N;
     double csum[D];   // this thread's share of the momentum sums per axis
@@ -584,13 +594,96 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
     __syncthreads();
 }
 
+// Columns per LDS tile of wg_nm_drift; its LDS: 4 P NMD_TC doubles.
+constexpr int NMD_TC = 16;
+
+// The normal-mode propagator's drift for a replica of distinguishable
+// particles (normal_modes_propagator.cpp:19-60; the reference allows it only
+// without exchange): zeroMomentum (cm, when fixcom) and B (p += dt/2 fx, the
+// physical forces), then the free ring polymer over dt in normal
+// modes: x_nm = C x and p_nm = C p, each mode rotated by its harmonic
+// propagator (the centroid, w = 0, moves freely), and back with C^T.
+// Columns (axis, particle) in tiles of NMD_TC staged in LDS; the arithmetic
+// is the single-system form's (k_nm_propagate), element for element.
+template <int D>
+__device__ __noinline__ void wg_nm_drift(const RepDesc& r_, const BatchPtrs& bp, const double (&cm_)[D], double* lds_) {
+    const RepDesc r = rep_uniform(r_);
+    double cm[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) cm[a] = uni(cm_[a]);
+    double* lds = uni(lds_);
+    const int N = r.N, P = r.P, ncols = D * N;
+    double* x = uni(bp.x) + r.off_x;
+    double* p = uni(bp.p) + r.off_x;
+    const double* f = uni(bp.fx) + r.off_x;      // physical forces
+    const double* Ct = uni(bp.Ct) + r.off_C;     // Ct[j*P + k] = C[k][j]
+    const double* CTt = uni(bp.CTt) + r.off_C;   // CTt[k*P + b] = CT[b][k]
+    const double* rot = uni(bp.rot) + (unsigned long long)uni((double*)r_.off_R);
+    const double half_dt = 0.5 * r.dt;
+    double* xc = lds;                          // [P][NMD_TC] Cartesian x, p of the tile's columns
+    double* pc = xc + (size_t)P * NMD_TC;
+    double* qn = pc + (size_t)P * NMD_TC;      // [P][NMD_TC] propagated modes
+    double* pn = qn + (size_t)P * NMD_TC;
+    for (int c0 = 0; c0 < ncols; c0 += NMD_TC) {
+        const int nc = min(NMD_TC, ncols - c0);
+        for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+            const int b = e / NMD_TC, c = e % NMD_TC;
+            double xv = 0.0, pv = 0.0;
+            if (c < nc) {
+                const size_t g = (size_t)b * ncols + c0 + c;
+                xv = x[g];
+                pv = p[g];
+                if (r.fixcom) pv -= cm[(c0 + c) / N];
+                pv += half_dt * f[g];
+            }
+            xc[e] = xv;
+            pc[e] = pv;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+            const int k = e / NMD_TC, c = e % NMD_TC;
+            double q = 0.0, pp = 0.0;
+            for (int j = 0; j < P; ++j) {
+                const double cj = Ct[(size_t)j * P + k];
+                q += cj * xc[j * NMD_TC + c];
+                pp += cj * pc[j * NMD_TC + c];
+            }
+            const double cs = rot[4 * k], sn = rot[4 * k + 1], mw = rot[4 * k + 2], w = rot[4 * k + 3];
+            if (w == 0) {
+                qn[e] = q + r.dt / r.mass * pp;
+                pn[e] = pp;
+            } else {
+                qn[e] = cs * q + sn / mw * pp;
+                pn[e] = (-1) * mw * sn * q + cs * pp;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+            const int b = e / NMD_TC, c = e % NMD_TC;
+            if (c >= nc) continue;
+            double xv = 0.0, pv = 0.0;
+            for (int k = 0; k < P; ++k) {
+                const double ck = CTt[(size_t)k * P + b];
+                xv += ck * qn[k * NMD_TC + c];
+                pv += ck * pn[k * NMD_TC + c];
+            }
+            const size_t g = (size_t)b * ncols + c0 + c;
+            x[g] = xv;
+            p[g] = pv;
+        }
+        __syncthreads();
+    }
+}
+
 // Physical forces and interior springs on every (bead, particle) of a replica
 // (simulation.cpp:353-455).  Each wave owns whole beads: the pairs of a bead
 // with Newton's third law (slice_pair_forces), then the trap and the springs.
 // after_exch_: the exchange phase has already put its forces into f of beads
 // 1 and P (only with the pair forces in LDS), which then get the physical
 // forces added instead of written.
-template <int D, int KIND>
+// NMP: replicas with the normal-mode propagator, whose physical forces also
+// go to fx (a separate instantiation, so the other replicas' code is unchanged).
+template <int D, int KIND, bool NMP = false>
 __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& bp, double* lds_, int after_exch_) {
     const RepDesc r = rep_uniform(r_);
     const bool after_exch = uni(after_exch_) != 0;
@@ -684,6 +777,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
                     spring = r.k * (dp + dn);
                 }
                 fo[e] = exchange_bead ? (after_exch ? fo[e] + phys : phys) : spring + phys;
+                if constexpr (NMP) uni(bp.fx)[r.off_x + bs * b + e] = phys;
             }
         }
     }
@@ -723,7 +817,9 @@ enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3
 // phase for timing, or the owed second B and O-step of replicas whose p is
 // pending (BM_FINISH; see wg_momenta).  The mode is a template parameter so
 // that every mode is its own kernel (and its own line in a kernel trace).
-template <int D, int KIND, int MODE>
+// NMP: a batch with normal-mode-propagator replicas (a separate
+// instantiation, so that the kernel of every other batch is unchanged).
+template <int D, int KIND, int MODE, bool NMP = false>
 static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
     extern __shared__ double smem[];
     double* red = smem;                 // [16]
@@ -736,8 +832,9 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         constexpr bool integ = MODE == BM_STEP || MODE == BM_INTEGRATOR;
         if constexpr (integ) {
             double cm[D];
-            wg_momenta<D>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
-            if (!r.fused) wg_kick_drift<D>(r, bp, cm);
+            wg_momenta<D, NMP>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
+            if (NMP && r.nmprop) wg_nm_drift<D>(r, bp, cm, lds);
+            else if (!r.fused) wg_kick_drift<D>(r, bp, cm);
         }
         // the second half of the grid (with two workgroups per compute unit,
         // mostly the other workgroup of each unit) takes the exchange before
@@ -748,14 +845,17 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if (xfirst)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
                            bp.status + ri, lds, r.xslots, 0);
-        if constexpr (MODE <= BM_PAIRS) wg_bead_forces<D, KIND>(r, bp, lds, xfirst ? 1 : 0);
+        if constexpr (MODE <= BM_PAIRS) {
+            if (NMP && r.nmprop) wg_bead_forces<D, KIND, true>(r, bp, lds, 0);   // (no exchange)
+            else wg_bead_forces<D, KIND>(r, bp, lds, xfirst ? 1 : 0);
+        }
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic && !xfirst)
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
                            bp.status + ri, lds, r.xslots, 1);
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];
-                wg_momenta<D>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
+                wg_momenta<D, NMP>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
             }
         }
         if constexpr (MODE == BM_FINISH) {

@@ -24,7 +24,8 @@ struct pimdb_batch {
     int threads = 1024, resident = 0;   // resident workgroups (0: all CUs)
     int cus = 0;
     int per_cu = 1;                     // workgroups that share one CU's LDS
-    size_t nx = 0, nV = 0, nC = 0;
+    size_t nx = 0, nV = 0, nC = 0, nR = 0;
+    double* d_fx = nullptr;             // physical forces of normal-mode-propagator replicas
     size_t lds = 0;
     BatchPtrs bp{};
     RepDesc* d_desc = nullptr;
@@ -35,6 +36,7 @@ struct pimdb_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool fused_any = false;       // some replica takes the fused normal-mode path
+    bool nmprop_any = false;      // some replica takes the normal-mode propagator (the NMP kernels)
     bool maybe_pending = false;   // a step ran since the last finishing launch
     std::string err;
     int fail(int code, const std::string& m) {
@@ -113,7 +115,8 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
             d.xslots = slots;
             need = std::max(need, x);
         }
-        const size_t nm = batch_nm_lds_doubles(p.nbeads, nw);
+        size_t nm = batch_nm_lds_doubles(p.nbeads, nw);
+        if (d.nmprop) nm = std::max(nm, 4 * (size_t)p.nbeads * NMD_TC);   // wg_nm_drift's tiles
         if (nm > avail) {
             *why = "replica " + std::to_string(r) + ": nbeads too large for the batched workgroup";
             return 0;
@@ -165,6 +168,14 @@ static int upload_schedule(pimdb_batch* b) {
 template <int D, int KIND>
 static void launch_batch_dk(pimdb_batch* b, int mode) {
     dim3 grid(b->grid), block(b->threads);
+    if (b->nmprop_any && (mode == BM_STEP || mode == BM_FORCES || mode == BM_INTEGRATOR)) {
+        switch (mode) {
+        case BM_STEP: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_STEP, true>), grid, block, b->lds, b->s, b->bp); break;
+        case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES, true>), grid, block, b->lds, b->s, b->bp); break;
+        default: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_INTEGRATOR, true>), grid, block, b->lds, b->s, b->bp); break;
+        }
+        return;
+    }
     switch (mode) {
     case BM_STEP: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_STEP>), grid, block, b->lds, b->s, b->bp); break;
     case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES>), grid, block, b->lds, b->s, b->bp); break;
@@ -244,10 +255,6 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             set_create_error("batched replicas use the Langevin thermostat or none");
             return PIMDB_E_ARG;
         }
-        if (reps[r].propagator != PIMDB_PROP_CARTESIAN) {
-            set_create_error("batched replicas use the cartesian propagator");
-            return PIMDB_E_ARG;
-        }
         if (reps[r].ndim != reps[0].ndim || reps[r].int_kind != reps[0].int_kind) {
             set_create_error("all replicas of a batch share ndim and the interaction kind");
             return PIMDB_E_ARG;
@@ -278,7 +285,9 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         RepDesc d{};
         d.N = p.natoms; d.P = p.nbeads; d.bosonic = dv.bosonic; d.pbc = p.pbc;
         d.thermostat = p.thermostat; d.nm = p.nmthermostat; d.fixcom = p.fixcom;
-        d.fused = (p.thermostat == PIMDB_THERMO_LANGEVIN && p.nmthermostat && p.nbeads <= NM_MFMA_MAXP) ? 1 : 0;
+        d.nmprop = p.propagator == PIMDB_PROP_NORMAL_MODES ? 1 : 0;   // (distinguishable only: validate_params)
+        d.fused = (p.thermostat == PIMDB_THERMO_LANGEVIN && p.nmthermostat && p.nbeads <= NM_MFMA_MAXP &&
+                   !d.nmprop) ? 1 : 0;
         if (d.fused) b->fused_any = true;
         d.L = p.size; d.invL = 1.0 / p.size; d.mass = p.mass; d.dt = p.dt;
         d.k = dv.k; d.bt = dv.bt; d.c1 = dv.c1; d.c2 = dv.c2;
@@ -288,6 +297,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
         d.off_V = b->nV; b->nV += (size_t)p.natoms + 1;
         d.off_C = b->nC; b->nC += (size_t)p.nbeads * p.nbeads;
+        d.off_R = b->nR; b->nR += 4 * (size_t)p.nbeads;
         b->desc.push_back(d);
         Nmax = std::max(Nmax, p.natoms);
     }
@@ -306,13 +316,18 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             hipMemset(*ptr, 0, std::max<size_t>(1, n) * sizeof(double)) != hipSuccess)
             rc = b->fail(PIMDB_E_HIP, "device allocation failed");
     };
-    double *Ct = nullptr, *CTt = nullptr;
+    double *Ct = nullptr, *CTt = nullptr, *rot = nullptr;
     alloc(&b->bp.x, b->nx); alloc(&b->bp.p, b->nx); alloc(&b->bp.f, b->nx);
     alloc(&b->bp.V, b->nV);
     alloc(&b->bp.cmp, 8 * (size_t)nrep);
     alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
+    alloc(&rot, b->nR);
+    for (const RepDesc& d : b->desc) b->nmprop_any = b->nmprop_any || d.nmprop;
+    if (b->nmprop_any) alloc(&b->d_fx, b->nx);
+    b->bp.fx = b->nmprop_any ? b->d_fx : b->bp.f;
     b->bp.Ct = Ct;
     b->bp.CTt = CTt;
+    b->bp.rot = rot;
     if (!rc && (hipMalloc((void**)&b->bp.status, sizeof(int) * nrep) != hipSuccess ||
                 hipMemset(b->bp.status, 0, sizeof(int) * nrep) != hipSuccess ||
                 hipMalloc((void**)&b->bp.ctl, 2 * sizeof(unsigned long long)) != hipSuccess ||
@@ -330,8 +345,13 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             transpose_square(P, C0.data(), hc.data() + b->desc[r].off_C);
             transpose_square(P, CT0.data(), hct.data() + b->desc[r].off_C);
         }
+        // the free ring polymer's rotation per mode (nm_rotation, as the
+        // single-system form builds it)
+        std::vector<double> hr(std::max<size_t>(1, b->nR));
+        for (int r = 0; r < nrep; ++r) nm_rotation(reps[r], derive(reps[r]), hr.data() + b->desc[r].off_R);
         if (hipMemcpy(Ct, hc.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(CTt, hct.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(CTt, hct.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(rot, hr.data(), sizeof(double) * b->nR, hipMemcpyHostToDevice) != hipSuccess)
             rc = b->fail(PIMDB_E_HIP, "upload of normal-mode tables failed");
     }
     if (!rc && (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
@@ -353,7 +373,8 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.cmp,
-                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt)};
+                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), const_cast<double*>(b->bp.rot),
+                      b->d_fx};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (b->bp.status) (void)hipFree(b->bp.status);
@@ -506,7 +527,7 @@ int pimdb_batch_time_step(pimdb_batch* b, int reps, double* ms_step) {
 int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) {
     if (!b || reps < 1 || !ms_out || phase < 0 || phase > 3) return PIMDB_E_ARG;
     static const int modes[4] = {BM_STEP, BM_PAIRS, BM_EXCHANGE, BM_INTEGRATOR};
-    double *sx = nullptr, *sp = nullptr, *sf = nullptr;
+    double *sx = nullptr, *sp = nullptr, *sf = nullptr, *sfx = nullptr;
     unsigned long long sctl[2];
     BCK(hipStreamSynchronize(b->s));
     BCK(hipMalloc((void**)&sx, sizeof(double) * b->nx));
@@ -515,6 +536,10 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
     BCK(hipMemcpy(sx, b->bp.x, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (b->d_fx) {   // the normal-mode replicas' physical forces
+        BCK(hipMalloc((void**)&sfx, sizeof(double) * b->nx));
+        BCK(hipMemcpy(sfx, b->d_fx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    }
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
     std::vector<double> scmp(8 * (size_t)b->R);
     BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
@@ -532,11 +557,13 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
     BCK(hipMemcpy(b->bp.x, sx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (sfx) BCK(hipMemcpy(b->d_fx, sfx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);
     (void)hipFree(sp);
     (void)hipFree(sf);
+    if (sfx) (void)hipFree(sfx);
     if (rc) return rc;
     return check_batch_status(b);
 }
@@ -546,7 +573,7 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
 // state, so the batch's trajectory is unaffected.
 int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int* resident_out) {
     if (!b || trial_steps < 1) return PIMDB_E_ARG;
-    double *sx = nullptr, *sp = nullptr, *sf = nullptr;
+    double *sx = nullptr, *sp = nullptr, *sf = nullptr, *sfx = nullptr;
     unsigned long long sctl[2];
     BCK(hipStreamSynchronize(b->s));
     BCK(hipMalloc((void**)&sx, sizeof(double) * b->nx));
@@ -555,6 +582,10 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(sx, b->bp.x, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sp, b->bp.p, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(sf, b->bp.f, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (b->d_fx) {   // the normal-mode replicas' physical forces
+        BCK(hipMalloc((void**)&sfx, sizeof(double) * b->nx));
+        BCK(hipMemcpy(sfx, b->d_fx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    }
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
     std::vector<double> scmp(8 * (size_t)b->R);
     BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
@@ -594,11 +625,13 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(b->bp.x, sx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (sfx) BCK(hipMemcpy(b->d_fx, sfx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);
     (void)hipFree(sp);
     (void)hipFree(sf);
+    if (sfx) (void)hipFree(sfx);
     if (threads_out) *threads_out = b->threads;
     if (resident_out) *resident_out = b->resident;
     return rc;

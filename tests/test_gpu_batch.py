@@ -1,6 +1,7 @@
 """Batched persistent form: every replica's forces against the oracle, every
 replica's trajectory against a single-system context with the same seed and
-state (device noise), graph determinism and the autotuner's state restore."""
+state (device noise; the Cartesian and the normal-mode propagator), graph
+determinism and the autotuner's state restore."""
 import numpy as np
 import pytest
 
@@ -82,10 +83,32 @@ def nm_replicas(n):
     return out
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "traps"])
+def nmprop_replicas(n):
+    """Distinguishable replicas with the normal-mode propagator (the reference
+    allows it only without exchange, params.cpp:146): normal-mode Langevin on
+    the matrix cores (P <= 64) and in the VALU form (P = 72), Cartesian
+    Langevin, no thermostat; with and without fixcom; uneven N, harmonic trap
+    and pairs."""
+    rng = np.random.default_rng(41)
+    out = []
+    for r in range(n):
+        N, P = int([8, 20, 33, 12, 17, 9][r % 6]), int([8, 24, 64, 72, 16, 4][r % 6])
+        th, nmth = [("langevin", 1), ("langevin", 1), ("langevin", 0), ("langevin", 1), ("none", 0),
+                    ("langevin", 0)][r % 6]
+        kw = dict(natoms=N, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+                  gamma=1.0 / 500.0, bosonic=0, pbc=0, size=30.0, ext_kind="harmonic", ext_omega=2.0 * MEV,
+                  int_kind="harmonic", int_omega=1.0 * MEV, cutoff=-1.0, propagator="normal_modes",
+                  thermostat=th, nmthermostat=nmth, fixcom=int(r % 3 != 1), seed=700 + r)
+        x = rng.uniform(-5.0, 5.0, size=(P, N, 3))
+        p = rng.normal(0, 0.5, size=(P, N, 3))
+        out.append((kw, x, p))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz", "traps", "nmprop"])
 def test_batch_forces_match_oracle(which):
     reps = {"dipole": lambda: dipole_replicas(12, seed=3), "aziz": lambda: aziz_replicas(6),
-            "traps": lambda: trap_replicas(6)}[which]()
+            "traps": lambda: trap_replicas(6), "nmprop": lambda: nmprop_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)
@@ -97,10 +120,10 @@ def test_batch_forces_match_oracle(which):
     b.close()
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "nm"])
+@pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop"])
 def test_batch_follows_single_system(which):
     reps = {"dipole": lambda: dipole_replicas(6, seed=5), "aziz": lambda: aziz_replicas(4),
-            "nm": lambda: nm_replicas(6)}[which]()
+            "nm": lambda: nm_replicas(6), "nmprop": lambda: nmprop_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     ctxs = []
     for r, (kw, x, p) in enumerate(reps):
@@ -156,6 +179,30 @@ def test_batch_autotune_restores_state():
     for s0, s1 in zip(before, after):
         assert all(np.array_equal(u, v) for u, v in zip(s0, s1))
     b.step(3)
+    b.close()
+
+
+def test_batch_autotune_then_nm_steps_follow_single_system():
+    """The autotuner's trial steps leave the state, the normal-mode replicas'
+    physical forces included, as they were: the steps after it follow the
+    single-system form."""
+    reps = nmprop_replicas(4)
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    b.autotune(trial_steps=2)
+    b.step(10)
+    for r, (kw, x, p) in enumerate(reps):
+        c = G.Context(G.make_params(**kw))
+        c.set_state(x, p)
+        c.forces()
+        c.step(10)
+        xb, pb, _ = b.state(r)
+        xc, pc, _ = c.state()
+        assert rel_err(xb, xc) < 1e-10, r
+        assert rel_err(pb, pc) < 1e-9, r
+        c.close()
     b.close()
 
 
