@@ -15,6 +15,7 @@
 
 #include "device_common.hpp"
 #include "exchange_batch.hpp"
+#include "nose_hoover.hpp"
 #include "pairs.hpp"
 
 namespace pimdb {
@@ -33,9 +34,20 @@ struct RepDesc {
     unsigned long long off_x;   // P*D*N block of x, p, f
     unsigned long long off_V;   // N+1 block of V
     unsigned long long off_C;   // P*P block of Ct, CTt
-    unsigned long long off_R;   // P*4 block of rot (nmprop)
+    // The fields below are read only by the normal-mode-propagator and Nose-Hoover paths (the EXT kernels);
+    // their arrays are reached from here rather than from BatchPtrs, whose copy in every other kernel's
+    // frame stays as it is.
     int nmprop;                 // normal-mode propagator (distinguishable particles): the kicks take the
                                 // physical forces fx, the drift is wg_nm_drift's
+    double* fx;                 // nmprop: this replica's physical forces [P][D][N]
+    const double* rot;          // nmprop: [P][4] cos(w dt), sin(w dt), m w, w of each mode
+    // Nose-Hoover chains (thermostat >= PIMDB_THERMO_NOSE_HOOVER, Cartesian coupling): chain length, masses
+    // Q_1 and Q_i, target ndof / beta~, beta~ (nose_hoover.cpp:10-32), and the replica's [P][groups][nc]
+    // block of eta (eta_dot and eta_dot_dot follow at nh_stride and 2 nh_stride)
+    int nh_nc;
+    double nh_Q1, nh_Qi, nh_required, nh_tbeta;
+    double* nh;
+    unsigned long long nh_stride;
 };
 
 struct BatchPtrs {
@@ -47,8 +59,6 @@ struct BatchPtrs {
     int* status;               // per replica: 1 when the exchange overflowed
     unsigned long long* ctl;   // [0] step counter, [1] finished-workgroup counter
     double* cmp;               // per replica [8]: pending COM momentum [0..3], known sum of p [4..7]
-    double* fx;                // physical forces (no springs) of nmprop replicas; f when there are none
-    const double* rot;         // per replica [P][4]: cos(w dt), sin(w dt), m w, w of each mode (nmprop)
 };
 
 // Sum over the workgroup of one double per thread; result in every thread.
@@ -82,9 +92,9 @@ __device__ __forceinline__ T* uni(T* p) {
 // Wave-uniform copy of a replica descriptor.  The phases below are kept out
 // of line (each one's registers would otherwise add to the others'); their
 // arguments arrive in vector registers, and these copies put the replica's
-// sizes, constants and offsets back into scalar registers.  (nmprop and
-// off_R are read where they are used: the copy is kept as the other phases
-// need it.)
+// sizes, constants and offsets back into scalar registers.  (The fields
+// of the normal-mode propagator and the Nose-Hoover chains are read where
+// they are used: the copy is kept as the other phases need it.)
 __device__ __forceinline__ RepDesc rep_uniform(const RepDesc& r) {
     RepDesc u;
     u.N = uni(r.N); u.P = uni(r.P); u.bosonic = uni(r.bosonic); u.pbc = uni(r.pbc);
@@ -177,9 +187,9 @@ __device__ __forceinline__ void wg_sub_com(double* p, int N, int P, const double
 // S the sum of p before the step (cmp[4..]: 0 after a deferred subtraction,
 // the sum of the state a caller set).  p then reads back through
 // pimdb_batch_get_state with the pending COM removed.
-// NMP: the kernel of a batch with normal-mode-propagator replicas (their
-// kicks take fx).
-template <int D, bool NMP = false>
+// EXT: the kernel of a batch with normal-mode-propagator or Nose-Hoover
+// replicas (the former's kicks take fx).
+template <int D, bool EXT = false>
 __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, bool kick_, uint32_t step_,
                                         uint32_t ostep_, double* red_, double* lds_, bool subtract_, double (&cm)[D],
                                         double* cmp_) {
@@ -193,7 +203,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     const int N = r.N, P = r.P;
     double* p = uni(bp.p) + r.off_x;
     // the kick's forces: the physical ones alone with the normal-mode propagator
-    const double* f = uni((NMP && uni(r_.nmprop)) ? (const double*)bp.fx : (const double*)bp.f) + r.off_x;
+    const double* f = (EXT && uni(r_.nmprop)) ? uni((const double*)r_.fx) : uni(bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
     const int ncols = D * N;
     double csum[D];   // this thread's share of the momentum sums per axis
@@ -554,6 +564,97 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     __syncthreads();
 }
 
+// One Nose-Hoover O-step of a replica with Cartesian coupling
+// (nose_hoover.cpp:96-150 per chain, as the single-system k_nh_ostep): the
+// optional half kick (p += dt/2 f), then for every bead the chains, one per
+// bead, per particle or per degree of freedom, each advanced on the kinetic
+// measure of its own degrees of freedom (nh_chain_step), and the momentum
+// scaling; then the centre-of-mass momentum of the result in cm (fixcom),
+// removed here when `subtract` (the single-system form removes it at the
+// start of the next O-step; nothing reads p in between).  Waves take beads.
+template <int D, bool EXT>
+__device__ __noinline__ void wg_nh_ostep(const RepDesc& r_, const BatchPtrs& bp, bool kick_, bool subtract_,
+                                         double* red_, double (&cm)[D]) {
+    const RepDesc r = rep_uniform(r_);
+    const bool kick = uni((int)kick_) != 0, subtract = uni((int)subtract_) != 0;
+    double* red = uni(red_);
+    const int N = r.N, P = r.P, kind = r.thermostat;
+    const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+    const int wave = uni((int)(threadIdx.x / WAVE));
+    double* p = uni(bp.p) + r.off_x;
+    const double* f = (EXT && uni(r_.nmprop)) ? uni((const double*)r_.fx) : uni(bp.f) + r.off_x;
+    NhArgs h{};
+    h.nc = uni(r_.nh_nc);
+    h.dt = r.dt;
+    h.Q1 = uni(r_.nh_Q1); h.Qi = uni(r_.nh_Qi); h.required = uni(r_.nh_required); h.thermo_beta = uni(r_.nh_tbeta);
+    const int groups = kind == PIMDB_THERMO_NOSE_HOOVER ? 1 : kind == PIMDB_THERMO_NOSE_HOOVER_NP ? N : N * D;
+    const unsigned long long nh_stride = (unsigned long long)uni((double*)r_.nh_stride);
+    double* eta = uni(r_.nh);
+    double* eta_dot = eta + nh_stride;
+    double* eta_dot_dot = eta_dot + nh_stride;
+    const double half_dt = 0.5 * r.dt;
+    double csum[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) csum[a] = 0.0;
+    for (int b = wave; b < P; b += nw) {
+        double* pb = p + (size_t)b * D * N;
+        const size_t cbase = (size_t)b * groups * h.nc;
+        if (kick) {
+            const double* fb = f + (size_t)b * D * N;
+            for (int e = lane; e < D * N; e += WAVE) pb[e] = pb[e] + half_dt * fb[e];
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (kind == PIMDB_THERMO_NOSE_HOOVER) {
+            double s = 0.0;
+            for (int e = lane; e < D * N; e += WAVE) s += pb[e] * pb[e];
+            s = wave_sum(s);
+            double sc = 0.0;
+            if (lane == 0) sc = nh_chain_step(h, eta + cbase, eta_dot + cbase, eta_dot_dot + cbase, s / r.mass);
+            sc = wave_bcast_uniform(sc, 0);
+            for (int e = lane; e < D * N; e += WAVE) pb[e] = pb[e] * sc;
+        } else if (kind == PIMDB_THERMO_NOSE_HOOVER_NP) {
+            for (int i = lane; i < N; i += WAVE) {
+                double ce = 0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) ce += pb[a * N + i] * pb[a * N + i];
+                ce /= r.mass;
+                const size_t o = cbase + (size_t)i * h.nc;
+                const double sc = nh_chain_step(h, eta + o, eta_dot + o, eta_dot_dot + o, ce);
+#pragma unroll
+                for (int a = 0; a < D; ++a) pb[a * N + i] = pb[a * N + i] * sc;
+            }
+        } else {
+            for (int e = lane; e < D * N; e += WAVE) {
+                const int a = e / N, i = e % N;
+                const double ce = pb[e] * pb[e] / r.mass;
+                const size_t o = cbase + ((size_t)i * D + a) * h.nc;
+                const double sc = nh_chain_step(h, eta + o, eta_dot + o, eta_dot_dot + o, ce);
+                pb[e] = pb[e] * sc;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int e = lane; e < D * N; e += WAVE) {
+            const double v = pb[e];
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                if (a == e / N) csum[a] += v;
+        }
+    }
+    __syncthreads();
+    if (r.fixcom) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) cm[a] = wg_sum(csum[a], red) / (double)(N * P);
+        if (subtract) {
+            __syncthreads();
+            wg_sub_com<D>(p, N, P, cm);
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < D; ++a) cm[a] = 0.0;
+    }
+    __syncthreads();
+}
+
 // zeroMomentum (cm, when fixcom), then B + A: p += dt/2 f, x += dt p / m
 // (simulation.cpp:581-603, velocity_verlet.cpp:24-38).
 template <int D>
@@ -615,10 +716,10 @@ __device__ __noinline__ void wg_nm_drift(const RepDesc& r_, const BatchPtrs& bp,
     const int N = r.N, P = r.P, ncols = D * N;
     double* x = uni(bp.x) + r.off_x;
     double* p = uni(bp.p) + r.off_x;
-    const double* f = uni(bp.fx) + r.off_x;      // physical forces
+    const double* f = uni((const double*)r_.fx);   // physical forces
     const double* Ct = uni(bp.Ct) + r.off_C;     // Ct[j*P + k] = C[k][j]
     const double* CTt = uni(bp.CTt) + r.off_C;   // CTt[k*P + b] = CT[b][k]
-    const double* rot = uni(bp.rot) + (unsigned long long)uni((double*)r_.off_R);
+    const double* rot = uni(r_.rot);
     const double half_dt = 0.5 * r.dt;
     double* xc = lds;                          // [P][NMD_TC] Cartesian x, p of the tile's columns
     double* pc = xc + (size_t)P * NMD_TC;
@@ -777,7 +878,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
                     spring = r.k * (dp + dn);
                 }
                 fo[e] = exchange_bead ? (after_exch ? fo[e] + phys : phys) : spring + phys;
-                if constexpr (NMP) uni(bp.fx)[r.off_x + bs * b + e] = phys;
+                if constexpr (NMP) uni(r_.fx)[bs * b + e] = phys;
             }
         }
     }
@@ -817,9 +918,10 @@ enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3
 // phase for timing, or the owed second B and O-step of replicas whose p is
 // pending (BM_FINISH; see wg_momenta).  The mode is a template parameter so
 // that every mode is its own kernel (and its own line in a kernel trace).
-// NMP: a batch with normal-mode-propagator replicas (a separate
-// instantiation, so that the kernel of every other batch is unchanged).
-template <int D, int KIND, int MODE, bool NMP = false>
+// EXT: a batch with normal-mode-propagator or Nose-Hoover replicas (a
+// separate instantiation, so that the kernel of every other batch is
+// unchanged).
+template <int D, int KIND, int MODE, bool EXT = false>
 static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
     extern __shared__ double smem[];
     double* red = smem;                 // [16]
@@ -832,8 +934,11 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         constexpr bool integ = MODE == BM_STEP || MODE == BM_INTEGRATOR;
         if constexpr (integ) {
             double cm[D];
-            wg_momenta<D, NMP>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
-            if (NMP && r.nmprop) wg_nm_drift<D>(r, bp, cm, lds);
+            if (EXT && r.thermostat >= PIMDB_THERMO_NOSE_HOOVER)
+                wg_nh_ostep<D, EXT>(r, bp, false, false, red, cm);
+            else
+                wg_momenta<D, EXT>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
+            if (EXT && r.nmprop) wg_nm_drift<D>(r, bp, cm, lds);
             else if (!r.fused) wg_kick_drift<D>(r, bp, cm);
         }
         // the second half of the grid (with two workgroups per compute unit,
@@ -846,7 +951,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
             wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
                            bp.status + ri, lds, r.xslots, 0);
         if constexpr (MODE <= BM_PAIRS) {
-            if (NMP && r.nmprop) wg_bead_forces<D, KIND, true>(r, bp, lds, 0);   // (no exchange)
+            if (EXT && r.nmprop) wg_bead_forces<D, KIND, true>(r, bp, lds, 0);   // (no exchange)
             else wg_bead_forces<D, KIND>(r, bp, lds, xfirst ? 1 : 0);
         }
         if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic && !xfirst)
@@ -855,7 +960,10 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];
-                wg_momenta<D, NMP>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
+                if (EXT && r.thermostat >= PIMDB_THERMO_NOSE_HOOVER)
+                    wg_nh_ostep<D, EXT>(r, bp, true, true, red, cm);
+                else
+                    wg_momenta<D, EXT>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
             }
         }
         if constexpr (MODE == BM_FINISH) {

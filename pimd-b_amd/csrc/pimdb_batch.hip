@@ -24,8 +24,10 @@ struct pimdb_batch {
     int threads = 1024, resident = 0;   // resident workgroups (0: all CUs)
     int cus = 0;
     int per_cu = 1;                     // workgroups that share one CU's LDS
-    size_t nx = 0, nV = 0, nC = 0, nR = 0;
+    size_t nx = 0, nV = 0, nC = 0, nR = 0, nH = 0;
     double* d_fx = nullptr;             // physical forces of normal-mode-propagator replicas
+    double* d_rot = nullptr;            // their rotation tables
+    double* d_nh = nullptr;             // Nose-Hoover chain states: eta, eta_dot, eta_dot_dot
     size_t lds = 0;
     BatchPtrs bp{};
     RepDesc* d_desc = nullptr;
@@ -36,7 +38,9 @@ struct pimdb_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool fused_any = false;       // some replica takes the fused normal-mode path
-    bool nmprop_any = false;      // some replica takes the normal-mode propagator (the NMP kernels)
+    bool nmprop_any = false;      // some replica takes the normal-mode propagator
+    bool nh_any = false;          // some replica takes Nose-Hoover chains
+    bool ext_any = false;         // either: the EXT kernels
     bool maybe_pending = false;   // a step ran since the last finishing launch
     std::string err;
     int fail(int code, const std::string& m) {
@@ -168,7 +172,7 @@ static int upload_schedule(pimdb_batch* b) {
 template <int D, int KIND>
 static void launch_batch_dk(pimdb_batch* b, int mode) {
     dim3 grid(b->grid), block(b->threads);
-    if (b->nmprop_any && (mode == BM_STEP || mode == BM_FORCES || mode == BM_INTEGRATOR)) {
+    if (b->ext_any && (mode == BM_STEP || mode == BM_FORCES || mode == BM_INTEGRATOR)) {
         switch (mode) {
         case BM_STEP: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_STEP, true>), grid, block, b->lds, b->s, b->bp); break;
         case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES, true>), grid, block, b->lds, b->s, b->bp); break;
@@ -251,8 +255,8 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             set_create_error("batched replicas use the Feldman-Hirshberg exchange");
             return PIMDB_E_ARG;
         }
-        if (reps[r].thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
-            set_create_error("batched replicas use the Langevin thermostat or none");
+        if (reps[r].thermostat >= PIMDB_THERMO_NOSE_HOOVER && reps[r].nmthermostat) {
+            set_create_error("batched Nose-Hoover replicas use Cartesian coupling (nmthermostat = 0)");
             return PIMDB_E_ARG;
         }
         if (reps[r].ndim != reps[0].ndim || reps[r].int_kind != reps[0].int_kind) {
@@ -279,6 +283,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     }
     b->cus = prop.multiProcessorCount;
     int Nmax = 1;
+    std::vector<size_t> off_R, off_H;   // per replica: its block of the rotation tables and of the chains
     for (int r = 0; r < nrep; ++r) {
         const pimdb_params& p = reps[r];
         const Derived dv = derive(p);
@@ -297,7 +302,22 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         d.off_x = b->nx; b->nx += (size_t)p.nbeads * p.ndim * p.natoms;
         d.off_V = b->nV; b->nV += (size_t)p.natoms + 1;
         d.off_C = b->nC; b->nC += (size_t)p.nbeads * p.nbeads;
-        d.off_R = b->nR; b->nR += 4 * (size_t)p.nbeads;
+        off_R.push_back(b->nR); b->nR += 4 * (size_t)p.nbeads;
+        off_H.push_back(b->nH);
+        if (p.thermostat >= PIMDB_THERMO_NOSE_HOOVER) {
+            // chains per bead and their degrees of freedom (nose_hoover.cpp:10-32, 152-163, 198-208)
+            const int groups = p.thermostat == PIMDB_THERMO_NOSE_HOOVER ? 1
+                               : p.thermostat == PIMDB_THERMO_NOSE_HOOVER_NP ? p.natoms : p.natoms * p.ndim;
+            const int ndof = p.thermostat == PIMDB_THERMO_NOSE_HOOVER ? p.ndim * p.natoms
+                             : p.thermostat == PIMDB_THERMO_NOSE_HOOVER_NP ? p.ndim : 1;
+            d.nh_nc = p.nchains;
+            d.nh_tbeta = dv.thermo_beta;            // beta / P
+            d.nh_Qi = 1.0 * 1.0 * dv.beta / p.nbeads;   // hbar^2 beta / P
+            d.nh_Q1 = ndof * d.nh_Qi;
+            d.nh_required = ndof / d.nh_tbeta;
+            b->nH += (size_t)p.nbeads * groups * p.nchains;
+            b->nh_any = true;
+        }
         b->desc.push_back(d);
         Nmax = std::max(Nmax, p.natoms);
     }
@@ -322,12 +342,20 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     alloc(&b->bp.cmp, 8 * (size_t)nrep);
     alloc(&Ct, b->nC + NM_BLOCK); alloc(&CTt, b->nC + NM_BLOCK);   // padded for block reads
     alloc(&rot, b->nR);
+    b->d_rot = rot;
     for (const RepDesc& d : b->desc) b->nmprop_any = b->nmprop_any || d.nmprop;
     if (b->nmprop_any) alloc(&b->d_fx, b->nx);
-    b->bp.fx = b->nmprop_any ? b->d_fx : b->bp.f;
+    if (b->nh_any) alloc(&b->d_nh, 3 * b->nH);   // the chains start at rest (alloc zeroes them)
+    for (int r = 0; r < nrep && !rc; ++r) {   // the EXT paths' per-replica arrays
+        RepDesc& d = b->desc[r];
+        d.fx = b->nmprop_any ? b->d_fx + d.off_x : nullptr;
+        d.rot = rot + off_R[r];
+        d.nh = b->nh_any ? b->d_nh + off_H[r] : nullptr;
+        d.nh_stride = b->nH;
+    }
+    b->ext_any = b->nmprop_any || b->nh_any;
     b->bp.Ct = Ct;
     b->bp.CTt = CTt;
-    b->bp.rot = rot;
     if (!rc && (hipMalloc((void**)&b->bp.status, sizeof(int) * nrep) != hipSuccess ||
                 hipMemset(b->bp.status, 0, sizeof(int) * nrep) != hipSuccess ||
                 hipMalloc((void**)&b->bp.ctl, 2 * sizeof(unsigned long long)) != hipSuccess ||
@@ -348,7 +376,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
         // the free ring polymer's rotation per mode (nm_rotation, as the
         // single-system form builds it)
         std::vector<double> hr(std::max<size_t>(1, b->nR));
-        for (int r = 0; r < nrep; ++r) nm_rotation(reps[r], derive(reps[r]), hr.data() + b->desc[r].off_R);
+        for (int r = 0; r < nrep; ++r) nm_rotation(reps[r], derive(reps[r]), hr.data() + off_R[r]);
         if (hipMemcpy(Ct, hc.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(CTt, hct.data(), sizeof(double) * b->nC, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(rot, hr.data(), sizeof(double) * b->nR, hipMemcpyHostToDevice) != hipSuccess)
@@ -373,8 +401,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.cmp,
-                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), const_cast<double*>(b->bp.rot),
-                      b->d_fx};
+                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), b->d_rot, b->d_fx, b->d_nh};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (b->bp.status) (void)hipFree(b->bp.status);
@@ -540,6 +567,11 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
         BCK(hipMalloc((void**)&sfx, sizeof(double) * b->nx));
         BCK(hipMemcpy(sfx, b->d_fx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     }
+    std::vector<double> seta;   // the Nose-Hoover chains
+    if (b->nh_any) {
+        seta.resize(3 * b->nH);
+        BCK(hipMemcpy(seta.data(), b->d_nh, sizeof(double) * 3 * b->nH, hipMemcpyDeviceToHost));
+    }
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
     std::vector<double> scmp(8 * (size_t)b->R);
     BCK(hipMemcpy(scmp.data(), b->bp.cmp, sizeof(double) * scmp.size(), hipMemcpyDeviceToHost));
@@ -558,6 +590,7 @@ int pimdb_batch_time_phase(pimdb_batch* b, int phase, int reps, double* ms_out) 
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     if (sfx) BCK(hipMemcpy(b->d_fx, sfx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (b->nh_any) BCK(hipMemcpy(b->d_nh, seta.data(), sizeof(double) * 3 * b->nH, hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);
@@ -585,6 +618,11 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     if (b->d_fx) {   // the normal-mode replicas' physical forces
         BCK(hipMalloc((void**)&sfx, sizeof(double) * b->nx));
         BCK(hipMemcpy(sfx, b->d_fx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    }
+    std::vector<double> seta;   // the Nose-Hoover chains
+    if (b->nh_any) {
+        seta.resize(3 * b->nH);
+        BCK(hipMemcpy(seta.data(), b->d_nh, sizeof(double) * 3 * b->nH, hipMemcpyDeviceToHost));
     }
     BCK(hipMemcpy(sctl, b->bp.ctl, sizeof(sctl), hipMemcpyDeviceToHost));
     std::vector<double> scmp(8 * (size_t)b->R);
@@ -626,6 +664,7 @@ int pimdb_batch_autotune(pimdb_batch* b, int trial_steps, int* threads_out, int*
     BCK(hipMemcpy(b->bp.p, sp, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     BCK(hipMemcpy(b->bp.f, sf, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
     if (sfx) BCK(hipMemcpy(b->d_fx, sfx, sizeof(double) * b->nx, hipMemcpyDeviceToDevice));
+    if (b->nh_any) BCK(hipMemcpy(b->d_nh, seta.data(), sizeof(double) * 3 * b->nH, hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.ctl, sctl, sizeof(sctl), hipMemcpyHostToDevice));
     BCK(hipMemcpy(b->bp.cmp, scmp.data(), sizeof(double) * scmp.size(), hipMemcpyHostToDevice));
     (void)hipFree(sx);

@@ -1,7 +1,8 @@
 """Batched persistent form: every replica's forces against the oracle, every
 replica's trajectory against a single-system context with the same seed and
-state (device noise; the Cartesian and the normal-mode propagator), graph
-determinism and the autotuner's state restore."""
+state (device noise; the Cartesian and the normal-mode propagator; Langevin,
+Nose-Hoover chains or no thermostat), graph determinism and the autotuner's
+state restore."""
 import numpy as np
 import pytest
 
@@ -105,10 +106,35 @@ def nmprop_replicas(n):
     return out
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "traps", "nmprop"])
+def nh_replicas(n):
+    """Nose-Hoover chain replicas with Cartesian coupling (nose_hoover.cpp):
+    one chain per bead, per particle and per degree of freedom, chain lengths
+    2 to 5, with and without fixcom, bosonic and distinguishable, the
+    Cartesian and (distinguishable) the normal-mode propagator; uneven N and
+    P, harmonic trap and pairs."""
+    rng = np.random.default_rng(51)
+    out = []
+    for r in range(n):
+        N, P = int([8, 20, 13, 12, 17, 9][r % 6]), int([8, 24, 16, 32, 4, 12][r % 6])
+        th = ["nose_hoover", "nose_hoover_np", "nose_hoover_np_dim"][r % 3]
+        bosonic = int(r % 2 == 0)
+        kw = dict(natoms=N, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+                  gamma=1.0 / 500.0, bosonic=bosonic, pbc=0, size=30.0, ext_kind="harmonic",
+                  ext_omega=2.0 * MEV, int_kind="harmonic", int_omega=1.0 * MEV, cutoff=-1.0,
+                  propagator="normal_modes" if (not bosonic and r % 4 == 1) else "cartesian",
+                  thermostat=th, nmthermostat=0, nchains=[4, 2, 5, 3, 4, 2][r % 6], fixcom=int(r % 3 != 1),
+                  seed=800 + r)
+        x = rng.uniform(-5.0, 5.0, size=(P, N, 3))
+        p = rng.normal(0, 0.5, size=(P, N, 3))
+        out.append((kw, x, p))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz", "traps", "nmprop", "nh"])
 def test_batch_forces_match_oracle(which):
     reps = {"dipole": lambda: dipole_replicas(12, seed=3), "aziz": lambda: aziz_replicas(6),
-            "traps": lambda: trap_replicas(6), "nmprop": lambda: nmprop_replicas(6)}[which]()
+            "traps": lambda: trap_replicas(6), "nmprop": lambda: nmprop_replicas(6),
+            "nh": lambda: nh_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)
@@ -120,10 +146,11 @@ def test_batch_forces_match_oracle(which):
     b.close()
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop"])
+@pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop", "nh"])
 def test_batch_follows_single_system(which):
     reps = {"dipole": lambda: dipole_replicas(6, seed=5), "aziz": lambda: aziz_replicas(4),
-            "nm": lambda: nm_replicas(6), "nmprop": lambda: nmprop_replicas(6)}[which]()
+            "nm": lambda: nm_replicas(6), "nmprop": lambda: nmprop_replicas(6),
+            "nh": lambda: nh_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     ctxs = []
     for r, (kw, x, p) in enumerate(reps):
@@ -184,9 +211,9 @@ def test_batch_autotune_restores_state():
 
 def test_batch_autotune_then_nm_steps_follow_single_system():
     """The autotuner's trial steps leave the state, the normal-mode replicas'
-    physical forces included, as they were: the steps after it follow the
-    single-system form."""
-    reps = nmprop_replicas(4)
+    physical forces and the Nose-Hoover chains included, as they were: the
+    steps after it follow the single-system form."""
+    reps = nmprop_replicas(3) + nh_replicas(3)
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)
@@ -204,6 +231,14 @@ def test_batch_autotune_then_nm_steps_follow_single_system():
         assert rel_err(pb, pc) < 1e-9, r
         c.close()
     b.close()
+
+
+def test_batch_rejects_nose_hoover_with_normal_mode_coupling():
+    kw = nh_replicas(1)[0][0]
+    kw["nmthermostat"] = 1
+    with pytest.raises(G.PimdbError, match="Cartesian coupling") as ei:
+        G.Batch([G.make_params(**kw)])
+    assert ei.value.code == G.E_ARG
 
 
 def test_batch_rejects_mixed_dimensions():
