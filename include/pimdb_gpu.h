@@ -214,9 +214,10 @@ int pimdb_exchange_vb_grad_noalloc(int natoms, int ndim, double beta_t, double s
 /* Batched replicas: many independent systems of uneven N, P and T advanced */
 /* by one persistent kernel per MD step (device noise), replayed from a     */
 /* hipGraph.  All replicas share ndim and the interaction kind; they take  */
-/* the Feldman-Hirshberg exchange, the Langevin thermostat or none, and     */
-/* either propagator (normal modes without exchange, as the reference);     */
-/* each replica equals one independent reference run                        */
+/* the Feldman-Hirshberg exchange, the Langevin thermostat, Nose-Hoover     */
+/* chains with Cartesian coupling or none, and either propagator (normal    */
+/* modes without exchange, as the reference); each replica equals one      */
+/* independent reference run                                                */
 /* (the reference has no batched form).  A replica draws the device noise  */
 /* stream of its seed, so it follows pimdb_step of a single-system context  */
 /* with the same seed and state up to summation order.                      */
