@@ -215,7 +215,7 @@ int pimdb_exchange_vb_grad_noalloc(int natoms, int ndim, double beta_t, double s
 /* by one persistent kernel per MD step (device noise), replayed from a     */
 /* hipGraph.  All replicas share ndim and the interaction kind; they take  */
 /* the Feldman-Hirshberg exchange, the Langevin thermostat, Nose-Hoover     */
-/* chains with Cartesian coupling or none, and either propagator (normal    */
+/* chains (either coupling) or none, and either propagator (normal          */
 /* modes without exchange, as the reference); each replica equals one      */
 /* independent reference run                                                */
 /* (the reference has no batched form).  A replica draws the device noise  */

@@ -48,6 +48,7 @@ struct RepDesc {
     double nh_Q1, nh_Qi, nh_required, nh_tbeta;
     double* nh;
     unsigned long long nh_stride;
+    double* modes;              // Nose-Hoover with normal-mode coupling: this replica's [P][D][N] modes
 };
 
 struct BatchPtrs {
@@ -144,6 +145,9 @@ __host__ __device__ inline size_t nm_mfma_lds_doubles(int P, int nwaves) {
 // STREAM_UNROLL elements per thread per pass and issue all their loads before
 // any store, so each thread keeps that many loads in flight.
 constexpr int STREAM_UNROLL = 4;
+// Columns per LDS tile of the normal-mode transforms of wg_nm_drift (LDS:
+// 4 P NMD_TC doubles) and wg_nh_ostep (P NMD_TC).
+constexpr int NMD_TC = 16;
 
 // Axis of element e of a [bead][axis][particle] array.
 template <int D>
@@ -564,17 +568,20 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
     __syncthreads();
 }
 
-// One Nose-Hoover O-step of a replica with Cartesian coupling
-// (nose_hoover.cpp:96-150 per chain, as the single-system k_nh_ostep): the
-// optional half kick (p += dt/2 f), then for every bead the chains, one per
-// bead, per particle or per degree of freedom, each advanced on the kinetic
-// measure of its own degrees of freedom (nh_chain_step), and the momentum
-// scaling; then the centre-of-mass momentum of the result in cm (fixcom),
+// One Nose-Hoover O-step of a replica (nose_hoover.cpp:96-150 per chain, as
+// the single-system k_nh_ostep): the optional half kick (p += dt/2 f), then
+// for every bead (Cartesian coupling) or every normal mode (normal-mode
+// coupling: the modes C p of the kicked momenta, with the single-system
+// k_momenta stage 1's arithmetic, and back with C^T after, stage 2's) the
+// chains, one per bead, per particle or per degree of freedom, each advanced
+// on the kinetic measure of its own degrees of freedom (nh_chain_step), and
+// the scaling; then the centre-of-mass momentum of the result in cm (fixcom),
 // removed here when `subtract` (the single-system form removes it at the
-// start of the next O-step; nothing reads p in between).  Waves take beads.
+// start of the next O-step; nothing reads p in between).  Waves take beads
+// (modes); the transforms take column tiles of NMD_TC in LDS.
 template <int D, bool EXT>
 __device__ __noinline__ void wg_nh_ostep(const RepDesc& r_, const BatchPtrs& bp, bool kick_, bool subtract_,
-                                         double* red_, double (&cm)[D]) {
+                                         double* red_, double* lds_, double (&cm)[D]) {
     const RepDesc r = rep_uniform(r_);
     const bool kick = uni((int)kick_) != 0, subtract = uni((int)subtract_) != 0;
     double* red = uni(red_);
@@ -593,13 +600,47 @@ __device__ __noinline__ void wg_nh_ostep(const RepDesc& r_, const BatchPtrs& bp,
     double* eta_dot = eta + nh_stride;
     double* eta_dot_dot = eta_dot + nh_stride;
     const double half_dt = 0.5 * r.dt;
+    const bool nmc = r.nm != 0;   // normal-mode coupling
+    const int ncols = D * N;
+    double* lds = uni(lds_);
+    const double* Ct = uni(bp.Ct) + r.off_C;     // Ct[j*P + k] = C[k][j]
+    const double* CTt = uni(bp.CTt) + r.off_C;   // CTt[k*P + b] = CT[b][k]
+    double* modes = nmc ? uni(r_.modes) : nullptr;
     double csum[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) csum[a] = 0.0;
+    if (nmc) {
+        // the modes of the kicked momenta
+        double* pc = lds;   // [P][NMD_TC]
+        for (int c0 = 0; c0 < ncols; c0 += NMD_TC) {
+            const int nc = min(NMD_TC, ncols - c0);
+            for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+                const int b = e / NMD_TC, c = e % NMD_TC;
+                double v = 0.0;
+                if (c < nc) {
+                    const size_t g = (size_t)b * ncols + c0 + c;
+                    v = p[g];
+                    if (kick) v += half_dt * f[g];
+                }
+                pc[e] = v;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+                const int k = e / NMD_TC, c = e % NMD_TC;
+                if (c < nc) {
+                    double sm = 0.0;
+                    for (int j = 0; j < P; ++j) sm += Ct[(size_t)j * P + k] * pc[j * NMD_TC + c];
+                    modes[(size_t)k * ncols + c0 + c] = sm;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    double* q = nmc ? modes : p;   // the momenta the chains scale: per bead, or per mode
     for (int b = wave; b < P; b += nw) {
-        double* pb = p + (size_t)b * D * N;
+        double* pb = q + (size_t)b * D * N;
         const size_t cbase = (size_t)b * groups * h.nc;
-        if (kick) {
+        if (kick && !nmc) {
             const double* fb = f + (size_t)b * D * N;
             for (int e = lane; e < D * N; e += WAVE) pb[e] = pb[e] + half_dt * fb[e];
             __builtin_amdgcn_wave_barrier();
@@ -633,14 +674,40 @@ __device__ __noinline__ void wg_nh_ostep(const RepDesc& r_, const BatchPtrs& bp,
             }
         }
         __builtin_amdgcn_wave_barrier();
-        for (int e = lane; e < D * N; e += WAVE) {
-            const double v = pb[e];
+        if (!nmc) {
+            for (int e = lane; e < D * N; e += WAVE) {
+                const double v = pb[e];
 #pragma unroll
-            for (int a = 0; a < D; ++a)
-                if (a == e / N) csum[a] += v;
+                for (int a = 0; a < D; ++a)
+                    if (a == e / N) csum[a] += v;
+            }
         }
     }
     __syncthreads();
+    if (nmc) {
+        // back to Cartesian momenta
+        double* pn = lds;   // [P][NMD_TC]
+        for (int c0 = 0; c0 < ncols; c0 += NMD_TC) {
+            const int nc = min(NMD_TC, ncols - c0);
+            for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+                const int k = e / NMD_TC, c = e % NMD_TC;
+                pn[e] = c < nc ? modes[(size_t)k * ncols + c0 + c] : 0.0;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < P * NMD_TC; e += blockDim.x) {
+                const int b = e / NMD_TC, c = e % NMD_TC;
+                if (c < nc) {
+                    double sm = 0.0;
+                    for (int k = 0; k < P; ++k) sm += CTt[(size_t)k * P + b] * pn[k * NMD_TC + c];
+                    p[(size_t)b * ncols + c0 + c] = sm;
+#pragma unroll
+                    for (int a = 0; a < D; ++a)
+                        if (a == (c0 + c) / N) csum[a] += sm;
+                }
+            }
+            __syncthreads();
+        }
+    }
     if (r.fixcom) {
 #pragma unroll
         for (int a = 0; a < D; ++a) cm[a] = wg_sum(csum[a], red) / (double)(N * P);
@@ -694,9 +761,6 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
     }
     __syncthreads();
 }
-
-// Columns per LDS tile of wg_nm_drift; its LDS: 4 P NMD_TC doubles.
-constexpr int NMD_TC = 16;
 
 // The normal-mode propagator's drift for a replica of distinguishable
 // particles (normal_modes_propagator.cpp:19-60; the reference allows it only
@@ -935,7 +999,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         if constexpr (integ) {
             double cm[D];
             if (EXT && r.thermostat >= PIMDB_THERMO_NOSE_HOOVER)
-                wg_nh_ostep<D, EXT>(r, bp, false, false, red, cm);
+                wg_nh_ostep<D, EXT>(r, bp, false, false, red, lds, cm);
             else
                 wg_momenta<D, EXT>(r, bp, false, step, 0, red, lds, false, cm, bp.cmp + 8 * (size_t)ri);
             if (EXT && r.nmprop) wg_nm_drift<D>(r, bp, cm, lds);
@@ -961,7 +1025,7 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];
                 if (EXT && r.thermostat >= PIMDB_THERMO_NOSE_HOOVER)
-                    wg_nh_ostep<D, EXT>(r, bp, true, true, red, cm);
+                    wg_nh_ostep<D, EXT>(r, bp, true, true, red, lds, cm);
                 else
                     wg_momenta<D, EXT>(r, bp, true, step + 1, 1, red, lds, true, cm, bp.cmp + 8 * (size_t)ri);
             }

@@ -28,6 +28,7 @@ struct pimdb_batch {
     double* d_fx = nullptr;             // physical forces of normal-mode-propagator replicas
     double* d_rot = nullptr;            // their rotation tables
     double* d_nh = nullptr;             // Nose-Hoover chain states: eta, eta_dot, eta_dot_dot
+    double* d_modes = nullptr;          // normal modes of Nose-Hoover replicas with normal-mode coupling
     size_t lds = 0;
     BatchPtrs bp{};
     RepDesc* d_desc = nullptr;
@@ -121,6 +122,8 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
         }
         size_t nm = batch_nm_lds_doubles(p.nbeads, nw);
         if (d.nmprop) nm = std::max(nm, 4 * (size_t)p.nbeads * NMD_TC);   // wg_nm_drift's tiles
+        if (p.thermostat >= PIMDB_THERMO_NOSE_HOOVER && p.nmthermostat)
+            nm = std::max(nm, (size_t)p.nbeads * NMD_TC);                 // wg_nh_ostep's tiles
         if (nm > avail) {
             *why = "replica " + std::to_string(r) + ": nbeads too large for the batched workgroup";
             return 0;
@@ -255,10 +258,6 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
             set_create_error("batched replicas use the Feldman-Hirshberg exchange");
             return PIMDB_E_ARG;
         }
-        if (reps[r].thermostat >= PIMDB_THERMO_NOSE_HOOVER && reps[r].nmthermostat) {
-            set_create_error("batched Nose-Hoover replicas use Cartesian coupling (nmthermostat = 0)");
-            return PIMDB_E_ARG;
-        }
         if (reps[r].ndim != reps[0].ndim || reps[r].int_kind != reps[0].int_kind) {
             set_create_error("all replicas of a batch share ndim and the interaction kind");
             return PIMDB_E_ARG;
@@ -346,12 +345,16 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     for (const RepDesc& d : b->desc) b->nmprop_any = b->nmprop_any || d.nmprop;
     if (b->nmprop_any) alloc(&b->d_fx, b->nx);
     if (b->nh_any) alloc(&b->d_nh, 3 * b->nH);   // the chains start at rest (alloc zeroes them)
+    bool nhnm_any = false;   // Nose-Hoover with normal-mode coupling: the modes' buffer
+    for (int r = 0; r < nrep; ++r) nhnm_any = nhnm_any || (reps[r].thermostat >= PIMDB_THERMO_NOSE_HOOVER && reps[r].nmthermostat);
+    if (nhnm_any) alloc(&b->d_modes, b->nx);
     for (int r = 0; r < nrep && !rc; ++r) {   // the EXT paths' per-replica arrays
         RepDesc& d = b->desc[r];
         d.fx = b->nmprop_any ? b->d_fx + d.off_x : nullptr;
         d.rot = rot + off_R[r];
         d.nh = b->nh_any ? b->d_nh + off_H[r] : nullptr;
         d.nh_stride = b->nH;
+        d.modes = nhnm_any ? b->d_modes + d.off_x : nullptr;
     }
     b->ext_any = b->nmprop_any || b->nh_any;
     b->bp.Ct = Ct;
@@ -401,7 +404,8 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.cmp,
-                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), b->d_rot, b->d_fx, b->d_nh};
+                      const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), b->d_rot, b->d_fx, b->d_nh,
+                      b->d_modes};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (b->bp.status) (void)hipFree(b->bp.status);

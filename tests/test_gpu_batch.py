@@ -1,8 +1,8 @@
 """Batched persistent form: every replica's forces against the oracle, every
 replica's trajectory against a single-system context with the same seed and
 state (device noise; the Cartesian and the normal-mode propagator; Langevin,
-Nose-Hoover chains or no thermostat), graph determinism and the autotuner's
-state restore."""
+Nose-Hoover chains with either coupling, or no thermostat), graph
+determinism and the autotuner's state restore."""
 import numpy as np
 import pytest
 
@@ -130,11 +130,23 @@ def nh_replicas(n):
     return out
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "traps", "nmprop", "nh"])
+def nhnm_replicas(n):
+    """nh_replicas with normal-mode coupling (the chains on the normal modes
+    of the momenta, thermostat_coupling.cpp)."""
+    out = []
+    for kw, x, p in nh_replicas(n):
+        kw = dict(kw)
+        kw["nmthermostat"] = 1
+        kw["seed"] += 100
+        out.append((kw, x, p))
+    return out
+
+
+@pytest.mark.parametrize("which", ["dipole", "aziz", "traps", "nmprop", "nh", "nhnm"])
 def test_batch_forces_match_oracle(which):
     reps = {"dipole": lambda: dipole_replicas(12, seed=3), "aziz": lambda: aziz_replicas(6),
             "traps": lambda: trap_replicas(6), "nmprop": lambda: nmprop_replicas(6),
-            "nh": lambda: nh_replicas(6)}[which]()
+            "nh": lambda: nh_replicas(6), "nhnm": lambda: nhnm_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)
@@ -146,11 +158,11 @@ def test_batch_forces_match_oracle(which):
     b.close()
 
 
-@pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop", "nh"])
+@pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop", "nh", "nhnm"])
 def test_batch_follows_single_system(which):
     reps = {"dipole": lambda: dipole_replicas(6, seed=5), "aziz": lambda: aziz_replicas(4),
             "nm": lambda: nm_replicas(6), "nmprop": lambda: nmprop_replicas(6),
-            "nh": lambda: nh_replicas(6)}[which]()
+            "nh": lambda: nh_replicas(6), "nhnm": lambda: nhnm_replicas(6)}[which]()
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     ctxs = []
     for r, (kw, x, p) in enumerate(reps):
@@ -213,7 +225,7 @@ def test_batch_autotune_then_nm_steps_follow_single_system():
     """The autotuner's trial steps leave the state, the normal-mode replicas'
     physical forces and the Nose-Hoover chains included, as they were: the
     steps after it follow the single-system form."""
-    reps = nmprop_replicas(3) + nh_replicas(3)
+    reps = nmprop_replicas(3) + nh_replicas(3) + nhnm_replicas(2)
     b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
     for r, (kw, x, p) in enumerate(reps):
         b.set_state(r, x, p)
@@ -231,14 +243,6 @@ def test_batch_autotune_then_nm_steps_follow_single_system():
         assert rel_err(pb, pc) < 1e-9, r
         c.close()
     b.close()
-
-
-def test_batch_rejects_nose_hoover_with_normal_mode_coupling():
-    kw = nh_replicas(1)[0][0]
-    kw["nmthermostat"] = 1
-    with pytest.raises(G.PimdbError, match="Cartesian coupling") as ei:
-        G.Batch([G.make_params(**kw)])
-    assert ei.value.code == G.E_ARG
 
 
 def test_batch_rejects_mixed_dimensions():
