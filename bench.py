@@ -304,7 +304,7 @@ def main():
     ap.add_argument("--no-phases", action="store_true", help="skip the per-phase timing launches")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs a ring polymer (or a set of "
+        print("bench.py measures one GPU: the benchmark runs a ring polymer (or a set of "
               "replicas) on a single MI355X; --gpus %d is not supported." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
