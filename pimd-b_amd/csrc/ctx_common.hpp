@@ -227,12 +227,23 @@ inline void launch_bead_energies_d(const pimdb_params& p, const Derived& d, cons
                                    hipStream_t s) {
     const ForceArgs fa = make_force_args(p, d);
     dim3 grid((p.natoms + FORCE_TILE - 1) / FORCE_TILE, p.nbeads);
-    const size_t lds = sizeof(double) * ((size_t)D * p.natoms + 3 * FORCE_WAVES);
+    const size_t red = sizeof(double) * 3 * FORCE_WAVES;
+    const size_t lds = sizeof(double) * (size_t)D * p.natoms + red;
+    const dim3 blk(FORCE_TILE * FORCE_WAVES);
+    if (lds > device_lds_limit()) {   // the slice read in place
+        switch (p.int_kind) {
+        case POT_HARMONIC: hipLaunchKernelGGL((k_bead_energies<D, POT_HARMONIC, true>), grid, blk, red, s, fa, x, part); break;
+        case POT_DIPOLE: hipLaunchKernelGGL((k_bead_energies<D, POT_DIPOLE, true>), grid, blk, red, s, fa, x, part); break;
+        case POT_AZIZ: hipLaunchKernelGGL((k_bead_energies<D, POT_AZIZ, true>), grid, blk, red, s, fa, x, part); break;
+        default: hipLaunchKernelGGL((k_bead_energies<D, POT_FREE, true>), grid, blk, red, s, fa, x, part); break;
+        }
+        return;
+    }
     switch (p.int_kind) {
-    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_energies<D, POT_HARMONIC>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
-    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_energies<D, POT_DIPOLE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
-    case POT_AZIZ: hipLaunchKernelGGL((k_bead_energies<D, POT_AZIZ>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
-    default: hipLaunchKernelGGL((k_bead_energies<D, POT_FREE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, x, part); break;
+    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_energies<D, POT_HARMONIC>), grid, blk, lds, s, fa, x, part); break;
+    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_energies<D, POT_DIPOLE>), grid, blk, lds, s, fa, x, part); break;
+    case POT_AZIZ: hipLaunchKernelGGL((k_bead_energies<D, POT_AZIZ>), grid, blk, lds, s, fa, x, part); break;
+    default: hipLaunchKernelGGL((k_bead_energies<D, POT_FREE>), grid, blk, lds, s, fa, x, part); break;
     }
 }
 

@@ -279,16 +279,16 @@ __device__ __forceinline__ Ring ring_stage(const RecShared& sh, int N, const dou
     for (int i = threadIdx.x; i <= N; i += blockDim.x) sh.lg[i] = i > 0 ? log((double)i) : 0.0;
     __syncthreads();
     const Ring r = make_ring(sh.x1, sh.xP, sh.Shi, sh.Slo, k, pbc, L);
-    if (threadIdx.x < WAVE) {
-        ring_prefix<D>(r, N, threadIdx.x, sh.Shi, sh.Slo);
-        if (Shi_out) {
-            for (int v = threadIdx.x; v < N; v += WAVE) {
-                Shi_out[v] = sh.Shi[v];
-                Slo_out[v] = sh.Slo[v];
-            }
+    if (threadIdx.x < WAVE) ring_prefix<D>(r, N, threadIdx.x, sh.Shi, sh.Slo);
+    __syncthreads();
+    // (after the barrier: the sums of other lanes' chunks are read, which in
+    // a device-memory workspace needs the workgroup's ordering)
+    if (Shi_out) {
+        for (int v = threadIdx.x; v < N; v += blockDim.x) {
+            Shi_out[v] = sh.Shi[v];
+            Slo_out[v] = sh.Slo[v];
         }
     }
-    __syncthreads();
     return r;
 }
 
@@ -518,7 +518,23 @@ __device__ void exchange_recursion(int N, double bt, const Ring& r, double* __re
 }
 
 // Block 0 runs the forward recursion (and writes the ring's prefix sums for
-// the exterior forces), block 1 the backward one.  x1, xP: SoA [a][i].
+// the exterior forces), block 1 the backward one, each on the workspace at
+// base (rec_shared_doubles(N, D, waves) doubles).  x1, xP: SoA [a][i].
+template <int D>
+__device__ __forceinline__ void exchange_recursions_on(double* base, int N, double bt, double k, int pbc, double L,
+                                                       const double* __restrict__ x1, const double* __restrict__ xP,
+                                                       double* __restrict__ V, double* __restrict__ Vb,
+                                                       double* __restrict__ Shi, double* __restrict__ Slo,
+                                                       int* __restrict__ status) {
+    const RecShared sh = rec_carve(base, N, D, blockDim.x / WAVE);
+    const bool fwd = blockIdx.x == 0;
+    const Ring r = ring_stage<D>(sh, N, x1, xP, k, pbc != 0, L, fwd ? Shi : nullptr, fwd ? Slo : nullptr);
+    if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+    else exchange_recursion<1, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+}
+
+// Both recursions with the workspace in LDS (rec_shared_doubles doubles of
+// dynamic shared memory).
 template <int D>
 static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, double bt, double k, int pbc, double L,
                                                                      const double* __restrict__ x1,
@@ -527,11 +543,26 @@ static __global__ void __launch_bounds__(1024) k_exchange_recursions(int N, doub
                                                                      double* __restrict__ Shi, double* __restrict__ Slo,
                                                                      int* __restrict__ status) {
     extern __shared__ double smem[];
-    const RecShared sh = rec_carve(smem, N, D, blockDim.x / WAVE);
-    const bool fwd = blockIdx.x == 0;
-    const Ring r = ring_stage<D>(sh, N, x1, xP, k, pbc != 0, L, fwd ? Shi : nullptr, fwd ? Slo : nullptr);
-    if (fwd) exchange_recursion<0, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
-    else exchange_recursion<1, D>(N, bt, r, V, Vb, status, sh, threadIdx.x, blockDim.x);
+    exchange_recursions_on<D>(smem, N, bt, k, pbc, L, x1, xP, V, Vb, Shi, Slo, status);
+}
+
+// The same for rings whose workspace exceeds a workgroup's LDS: block b's
+// workspace at ws + b rec_shared_doubles(N, D, waves), in device memory.  The
+// workspace is only touched between workgroup barriers or by the lane that
+// wrote it (the sequential passes read back their own stores), so the
+// workgroup's ordering of device memory is all the recursion relies on.
+template <int D>
+static __global__ void __launch_bounds__(1024) k_exchange_recursions_gws(int N, double bt, double k, int pbc,
+                                                                         double L, const double* __restrict__ x1,
+                                                                         const double* __restrict__ xP,
+                                                                         double* __restrict__ V,
+                                                                         double* __restrict__ Vb,
+                                                                         double* __restrict__ Shi,
+                                                                         double* __restrict__ Slo,
+                                                                         int* __restrict__ status,
+                                                                         double* __restrict__ ws) {
+    double* base = ws + (size_t)blockIdx.x * rec_shared_doubles(N, D, blockDim.x / WAVE);
+    exchange_recursions_on<D>(base, N, bt, k, pbc, L, x1, xP, V, Vb, Shi, Slo, status);
 }
 
 // Connection probability P[l][u] (quadratic_bosonic_exchange.cpp:142-157).

@@ -31,19 +31,27 @@ struct ForceArgs {
 constexpr int FORCE_TILE = 64;
 constexpr int FORCE_WAVES = 16;
 
-template <int D, int KIND>
+// GX: the slice is read in place from device memory instead of being staged
+// in LDS (slices larger than a workgroup's LDS); the LDS then holds only the
+// reduction.
+template <int D, int KIND, bool GX = false>
 static __global__ void __launch_bounds__(FORCE_TILE * FORCE_WAVES)
 k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f, double* __restrict__ fext) {
     extern __shared__ double smem[];
-    double* sx = smem;                                  // [D][N]
-    double* red = smem + (size_t)D * fa.N;             // [FORCE_WAVES-1][D][FORCE_TILE]
+    double* red = smem + (GX ? 0 : (size_t)D * fa.N);   // [FORCE_WAVES-1][D][FORCE_TILE]
     const int N = fa.N, P = fa.P;
     const int b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const size_t bs = (size_t)D * N;
     const double* xb = x + bs * b;
-    for (int e = tid; e < D * N; e += blockDim.x) sx[e] = xb[e];
-    __syncthreads();
+    const double* sx;                                   // [D][N]
+    if constexpr (GX) {
+        sx = xb;
+    } else {
+        for (int e = tid; e < D * N; e += blockDim.x) smem[e] = xb[e];
+        __syncthreads();
+        sx = smem;
+    }
 
     const int i = blockIdx.x * FORCE_TILE + lane;
     const bool pbc = fa.pbc != 0;
@@ -117,18 +125,24 @@ k_bead_forces(ForceArgs fa, const double* __restrict__ x, double* __restrict__ f
 // Pair and external potential energies and virial per bead, for the energy
 // observable (src/observables/energy.cpp:56-111): out[b] = {ext_pot, int_pot,
 // virial_sum} (un-normalised sums of the reference).
-template <int D, int KIND>
+// GX as for k_bead_forces.
+template <int D, int KIND, bool GX = false>
 static __global__ void __launch_bounds__(FORCE_TILE * FORCE_WAVES)
 k_bead_energies(ForceArgs fa, const double* __restrict__ x, double* __restrict__ part) {
     extern __shared__ double smem[];
-    double* sx = smem;
     const int N = fa.N;
     const int b = blockIdx.y;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const size_t bs = (size_t)D * N;
     const double* xb = x + bs * b;
-    for (int e = tid; e < D * N; e += blockDim.x) sx[e] = xb[e];
-    __syncthreads();
+    const double* sx;
+    if constexpr (GX) {
+        sx = xb;
+    } else {
+        for (int e = tid; e < D * N; e += blockDim.x) smem[e] = xb[e];
+        __syncthreads();
+        sx = smem;
+    }
     const int i = blockIdx.x * FORCE_TILE + lane;
     const bool pbc = fa.pbc != 0;
     double ext = 0.0, ipot = 0.0, vir = 0.0;
@@ -167,7 +181,7 @@ k_bead_energies(ForceArgs fa, const double* __restrict__ x, double* __restrict__
     ext = wave_sum(ext);
     ipot = wave_sum(ipot);
     vir = wave_sum(vir);
-    double* red = smem + (size_t)D * N;
+    double* red = smem + (GX ? 0 : (size_t)D * N);
     if (lane == 0) {
         red[wave * 3 + 0] = ext;
         red[wave * 3 + 1] = ipot;

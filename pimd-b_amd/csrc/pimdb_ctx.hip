@@ -39,6 +39,7 @@ struct pimdb_ctx {
     double *x = nullptr, *p = nullptr, *f = nullptr, *fext = nullptr;
     double *E = nullptr, *ET = nullptr, *V = nullptr, *Vb = nullptr, *prob = nullptr, *fex = nullptr;
     double *Shi = nullptr, *Slo = nullptr;   // link prefix sums of the ring (exterior forces)
+    double* xws = nullptr;   // the recursions' workspace in device memory, when it exceeds LDS
     double *C = nullptr, *CT = nullptr, *rot = nullptr, *com0 = nullptr, *com1 = nullptr;
     double *noise = nullptr, *obs = nullptr;
     size_t noise_cap = 0;
@@ -72,12 +73,23 @@ template <int D>
 static void launch_bead_forces_d(pimdb_ctx* ctx, hipStream_t s, double* f, double* fext) {
     ForceArgs fa = make_force_args(ctx->prm, ctx->dv);
     dim3 grid((ctx->N + FORCE_TILE - 1) / FORCE_TILE, ctx->P);
-    const size_t lds = sizeof(double) * ((size_t)D * ctx->N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE);
+    const size_t red = sizeof(double) * (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE;
+    const size_t lds = sizeof(double) * (size_t)D * ctx->N + red;
+    const dim3 blk(FORCE_TILE * FORCE_WAVES);
+    if (lds > device_lds_limit()) {   // the slice read in place
+        switch (ctx->prm.int_kind) {
+        case POT_HARMONIC: hipLaunchKernelGGL((k_bead_forces<D, POT_HARMONIC, true>), grid, blk, red, s, fa, ctx->x, f, fext); break;
+        case POT_DIPOLE: hipLaunchKernelGGL((k_bead_forces<D, POT_DIPOLE, true>), grid, blk, red, s, fa, ctx->x, f, fext); break;
+        case POT_AZIZ: hipLaunchKernelGGL((k_bead_forces<D, POT_AZIZ, true>), grid, blk, red, s, fa, ctx->x, f, fext); break;
+        default: hipLaunchKernelGGL((k_bead_forces<D, POT_FREE, true>), grid, blk, red, s, fa, ctx->x, f, fext); break;
+        }
+        return;
+    }
     switch (ctx->prm.int_kind) {
-    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_forces<D, POT_HARMONIC>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
-    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_forces<D, POT_DIPOLE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
-    case POT_AZIZ: hipLaunchKernelGGL((k_bead_forces<D, POT_AZIZ>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
-    default: hipLaunchKernelGGL((k_bead_forces<D, POT_FREE>), grid, dim3(FORCE_TILE * FORCE_WAVES), lds, s, fa, ctx->x, f, fext); break;
+    case POT_HARMONIC: hipLaunchKernelGGL((k_bead_forces<D, POT_HARMONIC>), grid, blk, lds, s, fa, ctx->x, f, fext); break;
+    case POT_DIPOLE: hipLaunchKernelGGL((k_bead_forces<D, POT_DIPOLE>), grid, blk, lds, s, fa, ctx->x, f, fext); break;
+    case POT_AZIZ: hipLaunchKernelGGL((k_bead_forces<D, POT_AZIZ>), grid, blk, lds, s, fa, ctx->x, f, fext); break;
+    default: hipLaunchKernelGGL((k_bead_forces<D, POT_FREE>), grid, blk, lds, s, fa, ctx->x, f, fext); break;
     }
 }
 
@@ -96,6 +108,14 @@ static void launch_exchange_forward(pimdb_ctx* ctx, hipStream_t s) {
     const int N = ctx->N, P = ctx->P, D = ctx->D;
     const double* x1 = ctx->x;
     const double* xP = ctx->x + (size_t)(P - 1) * D * N;
+    if (ctx->xws) {   // a ring too large for the LDS of one workgroup
+        switch (D) {
+        case 1: hipLaunchKernelGGL(k_exchange_recursions_gws<1>, dim3(2), dim3(1024), 0, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status, ctx->xws); break;
+        case 2: hipLaunchKernelGGL(k_exchange_recursions_gws<2>, dim3(2), dim3(1024), 0, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status, ctx->xws); break;
+        default: hipLaunchKernelGGL(k_exchange_recursions_gws<3>, dim3(2), dim3(1024), 0, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status, ctx->xws); break;
+        }
+        return;
+    }
     const size_t lds = rec_lds_bytes(N, D);
     switch (D) {
     case 1: hipLaunchKernelGGL(k_exchange_recursions<1>, dim3(2), dim3(1024), lds, s, N, ctx->dv.bt, ctx->dv.k, ctx->prm.pbc, ctx->prm.size, x1, xP, ctx->V, ctx->Vb, ctx->Shi, ctx->Slo, ctx->status); break;
@@ -354,6 +374,10 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     alloc(&ctx->Vb, sizeof(double) * (N + 1));
     alloc(&ctx->Shi, sizeof(double) * N);
     alloc(&ctx->Slo, sizeof(double) * N);
+    // both recursions' workspaces in device memory when one exceeds the LDS
+    // of a workgroup (the same kernel body; slower, no size limit of its own)
+    if (ctx->dv.bosonic && prm->exchange_algo != PIMDB_EXCHANGE_FACTORIAL && rec_lds_bytes(N, D) > device_lds_limit())
+        alloc(&ctx->xws, 2 * rec_lds_bytes(N, D));
     alloc(&ctx->prob, sizeof(double) * (size_t)N * N);
     alloc(&ctx->fex, sizeof(double) * 2 * (size_t)D * N);
     alloc(&ctx->C, sizeof(double) * (size_t)P * P);
@@ -406,10 +430,6 @@ int pimdb_ctx_create(const pimdb_params* prm, pimdb_ctx** out) {
     if (!rc && ctx->dv.bosonic && prm->exchange_algo == PIMDB_EXCHANGE_FACTORIAL &&
         sizeof(double) * factorial_lds_doubles(N, D) > device_lds_limit())
         rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the factorial exchange workgroup");
-    if (!rc && ctx->dv.bosonic && prm->exchange_algo != PIMDB_EXCHANGE_FACTORIAL && rec_lds_bytes(N, D) > device_lds_limit())
-        rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the exchange workgroup");
-    if (!rc && sizeof(double) * ((size_t)D * N + (size_t)(FORCE_WAVES - 1) * D * FORCE_TILE) > device_lds_limit())
-        rc = ctx->fail(PIMDB_E_ARG, "natoms too large for the force workgroup");
     if (rc) {
         set_create_error(ctx->err);
         pimdb_ctx_destroy(ctx);
@@ -427,7 +447,7 @@ void pimdb_ctx_destroy(pimdb_ctx* ctx) {
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     double* bufs[] = {ctx->x, ctx->p, ctx->f, ctx->fext, ctx->E, ctx->ET, ctx->V, ctx->Vb, ctx->prob, ctx->fex,
                       ctx->C, ctx->CT, ctx->rot, ctx->com0, ctx->com1, ctx->noise, ctx->obs, ctx->Shi, ctx->Slo,
-                      ctx->nh.eta, ctx->nh.eta_dot, ctx->nh.eta_dot_dot, ctx->modes};
+                      ctx->nh.eta, ctx->nh.eta_dot, ctx->nh.eta_dot_dot, ctx->modes, ctx->xws};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->status) (void)hipFree(ctx->status);
@@ -611,6 +631,8 @@ int pimdb_observables(pimdb_ctx* ctx, double* out) {
     } else if (ctx->dv.bosonic) {
         launch_exchange_forward(ctx, ctx->s0);
         launch_cycle_energies_ctx(ctx, ctx->s0);
+        if (sizeof(double) * (N + 1 + 16) > device_lds_limit())
+            return ctx->fail(PIMDB_E_ARG, "natoms too large for the estimator workgroup");
         hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
                            ctx->dv.bt, ctx->E, ctx->V, prim);
     }
@@ -643,6 +665,8 @@ int pimdb_exchange_estimators(pimdb_ctx* ctx, double* prim, double* prob_distinc
     } else {
         launch_exchange_forward(ctx, ctx->s0);
         launch_cycle_energies_ctx(ctx, ctx->s0);
+        if (sizeof(double) * (N + 1 + 16) > device_lds_limit())
+            return ctx->fail(PIMDB_E_ARG, "natoms too large for the estimator workgroup");
         hipLaunchKernelGGL(k_prim_estimator, dim3(1), dim3(1024), sizeof(double) * (N + 1 + 16), ctx->s0, N, P,
                            ctx->dv.bt, ctx->E, ctx->V, out);
     }

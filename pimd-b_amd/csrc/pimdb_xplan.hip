@@ -14,12 +14,19 @@ struct pimdb_xplan {
     int N = 0, D = 0, pbc = 0;
     double bt = 0.0, k = 0.0, L = 0.0;
     double *V = nullptr, *Vb = nullptr, *Shi = nullptr, *Slo = nullptr;
+    double* ws = nullptr;   // the recursions' workspace in device memory, when it exceeds LDS
     int* status = nullptr;
 };
 
 static int xplan_forward_internal(pimdb_xplan* pl, const double* x1, const double* xP, hipStream_t s) {
     const size_t lds = rec_lds_bytes(pl->N, pl->D);
-    switch (pl->D) {
+    if (pl->ws) {
+        switch (pl->D) {
+        case 1: hipLaunchKernelGGL(k_exchange_recursions_gws<1>, dim3(2), dim3(1024), 0, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status, pl->ws); break;
+        case 2: hipLaunchKernelGGL(k_exchange_recursions_gws<2>, dim3(2), dim3(1024), 0, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status, pl->ws); break;
+        default: hipLaunchKernelGGL(k_exchange_recursions_gws<3>, dim3(2), dim3(1024), 0, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status, pl->ws); break;
+        }
+    } else switch (pl->D) {
     case 1: hipLaunchKernelGGL(k_exchange_recursions<1>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
     case 2: hipLaunchKernelGGL(k_exchange_recursions<2>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
     default: hipLaunchKernelGGL(k_exchange_recursions<3>, dim3(2), dim3(1024), lds, s, pl->N, pl->bt, pl->k, pl->pbc, pl->L, x1, xP, pl->V, pl->Vb, pl->Shi, pl->Slo, pl->status); break;
@@ -44,14 +51,16 @@ int pimdb_xplan_create(int natoms, int ndim, double beta_t, double spring_k, int
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PIMDB_E_HIP;
-    if (rec_lds_bytes(natoms, ndim) > device_lds_limit()) return PIMDB_E_ARG;
     pimdb_xplan* pl = new pimdb_xplan();
     pl->N = natoms; pl->D = ndim; pl->pbc = pbc; pl->bt = beta_t; pl->k = spring_k; pl->L = size;
     if (hipMalloc((void**)&pl->Shi, natoms * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->Slo, natoms * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->V, (natoms + 1) * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&pl->Vb, (natoms + 1) * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&pl->status, sizeof(int)) != hipSuccess || hipMemset(pl->status, 0, sizeof(int)) != hipSuccess) {
+        hipMalloc((void**)&pl->status, sizeof(int)) != hipSuccess || hipMemset(pl->status, 0, sizeof(int)) != hipSuccess ||
+        // a ring whose recursion workspace exceeds a workgroup's LDS runs on device memory
+        (rec_lds_bytes(natoms, ndim) > device_lds_limit() &&
+         hipMalloc((void**)&pl->ws, 2 * rec_lds_bytes(natoms, ndim)) != hipSuccess)) {
         pimdb_xplan_destroy(pl);
         return PIMDB_E_HIP;
     }
@@ -67,6 +76,7 @@ void pimdb_xplan_destroy(pimdb_xplan* pl) {
     if (pl->V) (void)hipFree(pl->V);
     if (pl->Vb) (void)hipFree(pl->Vb);
     if (pl->status) (void)hipFree(pl->status);
+    if (pl->ws) (void)hipFree(pl->ws);
     delete pl;
 }
 
