@@ -239,8 +239,9 @@ int pimdb_batch_step(pimdb_batch* batch, int nsteps);
 /* Workgroup size of the persistent kernel (256, 512 or 1024; 0 keeps the
  * current one) and number of workgroups (0 keeps the current one; default
  * one per compute unit).  More workgroups than compute units share each CU's
- * LDS, and every replica's phases are planned for that share; PIMDB_E_ARG
- * (the previous shape kept) when some replica does not fit it. */
+ * LDS, and every replica's phases are planned for that share (an exchange
+ * workspace that does not fit goes to device memory, allocated here);
+ * PIMDB_E_ARG (the previous shape kept) when some replica does not fit it. */
 int pimdb_batch_configure(pimdb_batch* batch, int threads, int resident);
 /* Times trial_steps steps for each launch shape (256, 512 or 1024 threads,
  * 1, 2 or 4 workgroups per compute unit, at most 1024 threads per CU) whose

@@ -39,7 +39,13 @@ struct RepDesc {
     // frame stays as it is.
     int nmprop;                 // normal-mode propagator (distinguishable particles): the kicks take the
                                 // physical forces fx, the drift is wg_nm_drift's
-    double* fx;                 // nmprop: this replica's physical forces [P][D][N]
+    union {
+        double* fx;             // nmprop: this replica's physical forces [P][D][N]
+        // Bosonic replicas (never nmprop: the normal-mode propagator is for distinguishable particles): the
+        // exchange's workspace in device memory (xb_layout with one chunk slot per wave) when the layout does
+        // not fit the workgroup's LDS share, else null; a batch with such a replica runs the EXT kernels.
+        double* xws;
+    };
     const double* rot;          // nmprop: [P][4] cos(w dt), sin(w dt), m w, w of each mode
     // Nose-Hoover chains (thermostat >= PIMDB_THERMO_NOSE_HOOVER, Cartesian coupling): chain length, masses
     // Q_1 and Q_i, target ndof / beta~, beta~ (nose_hoover.cpp:10-32), and the replica's [P][groups][nc]
@@ -959,7 +965,11 @@ __host__ __device__ inline size_t batch_nm_lds_doubles(int P, int nwaves) {
 // the backward recursion in linear form with the exterior forces added into f
 // of beads 1 and P (add_ = 0: written into them).  Kept out of line (its registers would otherwise add to
 // every other phase's); the arguments are made wave-uniform on entry.
-template <int D>
+// DEV: the workspace lds_ is the replica's device-memory one (RepDesc::xws), a
+// separate instantiation so that the LDS one keeps its LDS instructions.  The
+// workspace is only touched between workgroup barriers or by the lane that
+// wrote it, so the workgroup's ordering of device memory is all it needs.
+template <int D, bool DEV = false>
 __device__ __noinline__ void wg_exchange(int N_, int P_, int pbc_, double L_, double k_, double bt_, const double* x_,
                                          double* f_, double* V_, int* status_, double* lds_, int slots_, int add_) {
     const int N = uni(N_), P = uni(P_);
@@ -982,9 +992,9 @@ enum BatchMode : int { BM_STEP = 0, BM_FORCES = 1, BM_PAIRS = 2, BM_EXCHANGE = 3
 // phase for timing, or the owed second B and O-step of replicas whose p is
 // pending (BM_FINISH; see wg_momenta).  The mode is a template parameter so
 // that every mode is its own kernel (and its own line in a kernel trace).
-// EXT: a batch with normal-mode-propagator or Nose-Hoover replicas (a
-// separate instantiation, so that the kernel of every other batch is
-// unchanged).
+// EXT: a batch with normal-mode-propagator or Nose-Hoover replicas, or with
+// an exchange workspace in device memory (a separate instantiation, so that
+// the kernel of every other batch is unchanged).
 template <int D, int KIND, int MODE, bool EXT = false>
 static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
     extern __shared__ double smem[];
@@ -1011,16 +1021,26 @@ static __global__ void __launch_bounds__(1024) k_batch_step(BatchPtrs bp) {
         // walk instead of its own exchange
         const bool xfirst = MODE <= BM_FORCES && r.bosonic && r.plds && KIND != POT_FREE && r.cutoff != 0.0 &&
                             2 * (int)blockIdx.x >= (int)gridDim.x;   // (pair forces in LDS: f written once)
-        if (xfirst)
-            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.status + ri, lds, r.xslots, 0);
+        if (xfirst) {
+            if (EXT && r.bosonic && r.xws)
+                wg_exchange<D, true>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                                     bp.status + ri, r.xws, r.xslots, 0);
+            else
+                wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                               bp.status + ri, lds, r.xslots, 0);
+        }
         if constexpr (MODE <= BM_PAIRS) {
             if (EXT && r.nmprop) wg_bead_forces<D, KIND, true>(r, bp, lds, 0);   // (no exchange)
             else wg_bead_forces<D, KIND>(r, bp, lds, xfirst ? 1 : 0);
         }
-        if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic && !xfirst)
-            wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
-                           bp.status + ri, lds, r.xslots, 1);
+        if ((MODE <= BM_FORCES || MODE == BM_EXCHANGE) && r.bosonic && !xfirst) {
+            if (EXT && r.bosonic && r.xws)
+                wg_exchange<D, true>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                                     bp.status + ri, r.xws, r.xslots, 1);
+            else
+                wg_exchange<D>(r.N, r.P, r.pbc, r.L, r.k, r.bt, bp.x + r.off_x, bp.f + r.off_x, bp.V + r.off_V,
+                               bp.status + ri, lds, r.xslots, 1);
+        }
         if constexpr (integ) {
             if (!r.fused) {   // fused replicas owe it to the next step's first O-step
                 double cm[D];

@@ -29,6 +29,8 @@ struct pimdb_batch {
     double* d_rot = nullptr;            // their rotation tables
     double* d_nh = nullptr;             // Nose-Hoover chain states: eta, eta_dot, eta_dot_dot
     double* d_modes = nullptr;          // normal modes of Nose-Hoover replicas with normal-mode coupling
+    double* d_xws = nullptr;            // exchange workspaces in device memory (RepDesc::xws)
+    size_t xws_cap = 0;                 // doubles held by d_xws
     size_t lds = 0;
     BatchPtrs bp{};
     RepDesc* d_desc = nullptr;
@@ -41,7 +43,8 @@ struct pimdb_batch {
     bool fused_any = false;       // some replica takes the fused normal-mode path
     bool nmprop_any = false;      // some replica takes the normal-mode propagator
     bool nh_any = false;          // some replica takes Nose-Hoover chains
-    bool ext_any = false;         // either: the EXT kernels
+    bool xdev_any = false;        // some replica's exchange workspace is in device memory
+    bool ext_any = false;         // any of the three: the EXT kernels
     bool maybe_pending = false;   // a step ran since the last finishing launch
     std::string err;
     int fail(int code, const std::string& m) {
@@ -89,9 +92,13 @@ static void build_schedule(pimdb_batch* b, std::vector<int>& sched, std::vector<
 // LDS plan of every replica for `threads` threads per workgroup and
 // `per_cu` workgroups per compute unit (each then gets 1/per_cu of its LDS):
 // whether the pair forces of every wave's slice stay in LDS, how many chunk
-// slots the exchange's backward pass gets, and the largest need.  Returns the dynamic LDS bytes, or 0
+// slots the exchange's backward pass gets, and the largest need.  A replica
+// whose exchange layout does not fit even with one chunk slot gets its
+// workspace in device memory instead (one slot per wave; its size in doubles
+// into (*xdev)[r], 0 for the others).  Returns the dynamic LDS bytes, or 0
 // (with *why) when some replica does not fit.
-static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepDesc>& desc, std::string* why) {
+static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepDesc>& desc, std::string* why,
+                       std::vector<size_t>* xdev = nullptr) {
     const size_t budget = device_lds_limit() / (size_t)per_cu / sizeof(double);   // doubles
     const size_t avail = budget > 16 ? budget - 16 : 0;                             // after the reduction slots
     const int nw = threads / WAVE;
@@ -101,6 +108,7 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
         const int N = p.natoms, D = p.ndim;
         RepDesc& d = desc[r];
         d.xslots = 1;
+        if (xdev) (*xdev)[r] = 0;
         if (d.bosonic) {
             // the exchange's layout with as many chunk slots (up to one per
             // wave) as fit
@@ -112,10 +120,10 @@ static size_t plan_lds(pimdb_batch* b, int threads, int per_cu, std::vector<RepD
             int slots = nw;
             size_t x = (size_t)xb_layout(N, D, nw, slots).total;
             while (slots > 1 && x > avail) x = (size_t)xb_layout(N, D, nw, --slots).total;
-            if (x > avail) {
-                *why = "replica " + std::to_string(r) + ": natoms too large for the exchange workspace of a "
-                       "batched workgroup";
-                return 0;
+            if (x > avail) {   // the workspace in device memory
+                slots = nw;
+                if (xdev) (*xdev)[r] = (size_t)xb_layout(N, D, nw, slots).total;
+                x = 0;
             }
             d.xslots = slots;
             need = std::max(need, x);
@@ -146,8 +154,27 @@ static int upload_schedule(pimdb_batch* b) {
     {
         std::vector<RepDesc> desc = b->desc;
         std::string why;
-        const size_t lds = plan_lds(b, b->threads, b->per_cu, desc, &why);
+        std::vector<size_t> xdev(b->R, 0);
+        const size_t lds = plan_lds(b, b->threads, b->per_cu, desc, &why, &xdev);
         if (!lds) return b->fail(PIMDB_E_ARG, why);
+        // the device-memory exchange workspaces of this plan
+        size_t total = 0;
+        for (size_t n : xdev) total += n;
+        if (total > b->xws_cap) {
+            BCK(hipStreamSynchronize(b->s));
+            if (b->d_xws) (void)hipFree(b->d_xws);
+            b->d_xws = nullptr;
+            b->xws_cap = 0;
+            BCK(hipMalloc((void**)&b->d_xws, total * sizeof(double)));
+            b->xws_cap = total;
+        }
+        size_t o = 0;
+        for (int r = 0; r < b->R; ++r) {
+            if (desc[r].bosonic) desc[r].xws = xdev[r] ? b->d_xws + o : nullptr;   // (fx is the nmprop one's)
+            o += xdev[r];
+        }
+        b->xdev_any = total > 0;
+        b->ext_any = b->nmprop_any || b->nh_any || b->xdev_any;
         b->desc = desc;
         b->lds = lds;
         if (b->d_desc)
@@ -175,10 +202,11 @@ static int upload_schedule(pimdb_batch* b) {
 template <int D, int KIND>
 static void launch_batch_dk(pimdb_batch* b, int mode) {
     dim3 grid(b->grid), block(b->threads);
-    if (b->ext_any && (mode == BM_STEP || mode == BM_FORCES || mode == BM_INTEGRATOR)) {
+    if (b->ext_any && (mode == BM_STEP || mode == BM_FORCES || mode == BM_INTEGRATOR || mode == BM_EXCHANGE)) {
         switch (mode) {
         case BM_STEP: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_STEP, true>), grid, block, b->lds, b->s, b->bp); break;
         case BM_FORCES: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_FORCES, true>), grid, block, b->lds, b->s, b->bp); break;
+        case BM_EXCHANGE: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_EXCHANGE, true>), grid, block, b->lds, b->s, b->bp); break;
         default: hipLaunchKernelGGL((k_batch_step<D, KIND, BM_INTEGRATOR, true>), grid, block, b->lds, b->s, b->bp); break;
         }
         return;
@@ -350,7 +378,7 @@ int pimdb_batch_create(const pimdb_params* reps, int nrep, pimdb_batch** out) {
     if (nhnm_any) alloc(&b->d_modes, b->nx);
     for (int r = 0; r < nrep && !rc; ++r) {   // the EXT paths' per-replica arrays
         RepDesc& d = b->desc[r];
-        d.fx = b->nmprop_any ? b->d_fx + d.off_x : nullptr;
+        d.fx = d.nmprop ? b->d_fx + d.off_x : nullptr;   // (shares its slot with xws, bosonic replicas')
         d.rot = rot + off_R[r];
         d.nh = b->nh_any ? b->d_nh + off_H[r] : nullptr;
         d.nh_stride = b->nH;
@@ -405,7 +433,7 @@ void pimdb_batch_destroy(pimdb_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     double* bufs[] = {b->bp.x, b->bp.p, b->bp.f, b->bp.V, b->bp.cmp,
                       const_cast<double*>(b->bp.Ct), const_cast<double*>(b->bp.CTt), b->d_rot, b->d_fx, b->d_nh,
-                      b->d_modes};
+                      b->d_modes, b->d_xws};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (b->bp.status) (void)hipFree(b->bp.status);

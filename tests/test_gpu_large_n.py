@@ -6,7 +6,11 @@ the C ABI:
   D=2 on a 160 KB workgroup), through the context and through the exchange
   plan;
 * the pair forces and pair energies reading their slice in place (the
-  staged slice holds 3N + 2880 doubles at D=3: N above 5866).
+  staged slice holds 3N + 2880 doubles at D=3: N above 5866);
+* batched replicas whose exchange layout exceeds the workgroup's LDS share
+  (about 15N + 3000 doubles at D=3: N above about 470 with two 512-thread
+  workgroups per CU, 1080 with one 1024-thread workgroup), whose exchange
+  then runs on a workspace in device memory.
 
 The exchange is judged against the exact exchange of the same double inputs
 (oracle.exchange_quad, __float128), as the N=1024 rings are: these rings are
@@ -21,7 +25,7 @@ import pytest
 
 import oracle as O
 import pimdb_amd as G
-from helpers import aziz_config, rel_err
+from helpers import aziz_config, per_bead_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +180,48 @@ def test_pair_forces_and_energies_slice_in_place():
         assert abs(og[j] - oo[j]) <= 1e-9 * max(abs(oo[j]), 1e-12), (name, og[j], oo[j])
     for j in (8, 9):
         assert abs(og[j] - oo[j]) <= 1e-10 + 1e-9 * abs(oo[j]), (names[j], og[j], oo[j])
+
+
+@pytest.mark.parametrize("shape", ["512x2", "1024x1"])
+def test_batch_exchange_device_workspace(shape):
+    """N=256 (LDS) beside a replica whose exchange runs on device memory
+    (N=600 on two 512-thread workgroups per CU, N=1200 on one 1024-thread
+    workgroup), P=4, Aziz with PBC: every replica's forces against the oracle
+    per bead at 1e-10, then 10 device-noise steps against the single-system
+    context (x to 1e-10, p and forces to 1e-9, as the batched tests of the
+    other paths)."""
+    cus = G.device_cus()
+    threads, resident, Nbig = (512, 2 * cus, 600) if shape == "512x2" else (1024, cus, 1200)
+    reps = []
+    for r, N in enumerate([256, Nbig]):
+        kw, x = aziz_config(N, 4, 1.5, replicate=False, seed=700 + r)
+        kw["seed"] = 31 + r
+        p = np.random.default_rng(70 + r).normal(0, 0.3, size=x.shape)
+        reps.append((kw, x, p))
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    b.configure(threads, resident)
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    errs = []
+    for r, (kw, x, p) in enumerate(reps):
+        errs.append(per_bead_err(b.state(r)[2], O.update_forces(O.make_params(**kw), x)))
+    print(shape, "per-replica force errors", errs, flush=True)
+    assert max(errs) < 1e-10, errs
+    ctxs = []
+    for kw, x, p in reps:
+        c = G.Context(G.make_params(**kw))
+        c.set_state(x, p)
+        c.forces()
+        ctxs.append(c)
+    b.step(10)
+    for r, c in enumerate(ctxs):
+        c.step(10)
+        xb, pb, fb = b.state(r)
+        xc, pc, fc = c.state()
+        print(shape, r, "x %.2e p %.2e f %.2e" % (rel_err(xb, xc), rel_err(pb, pc), per_bead_err(fb, fc)), flush=True)
+        assert rel_err(xb, xc) < 1e-10, r
+        assert rel_err(pb, pc) < 1e-9, r
+        assert per_bead_err(fb, fc) < 1e-9, r
+        c.close()
+    b.close()
