@@ -25,7 +25,7 @@ import pytest
 
 import oracle as O
 import pimdb_amd as G
-from helpers import aziz_config, per_bead_err, rel_err
+from helpers import AMU, KELVIN, MEV, aziz_config, per_bead_err, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -220,6 +220,59 @@ def test_batch_exchange_device_workspace(shape):
         xb, pb, fb = b.state(r)
         xc, pc, fc = c.state()
         print(shape, r, "x %.2e p %.2e f %.2e" % (rel_err(xb, xc), rel_err(pb, pc), per_bead_err(fb, fc)), flush=True)
+        assert rel_err(xb, xc) < 1e-10, r
+        assert rel_err(pb, pc) < 1e-9, r
+        assert per_bead_err(fb, fc) < 1e-9, r
+        c.close()
+    b.close()
+
+
+def lattice_dipole_ring(N, P, T_kelvin, seed, nmthermostat):
+    """D=2 He-4 with a 3 meV trap and dipole pairs (all pairs), particles on a
+    jittered 4-bohr lattice (at least 3 bohr apart) and beads 0.3 bohr about
+    them, as the batched form's config-4 replicas."""
+    rng = np.random.default_rng(seed)
+    kw = dict(natoms=N, nbeads=P, ndim=2, mass=4.0026 * AMU, temperature=T_kelvin * KELVIN, dt=20.0,
+              gamma=1.0 / 2000.0, bosonic=1, pbc=0, size=60.0, ext_kind="harmonic", ext_omega=3.0 * MEV,
+              int_kind="dipole", int_strength=1.3, cutoff=-1.0, propagator="cartesian", thermostat="langevin",
+              nmthermostat=nmthermostat, fixcom=1, seed=seed)
+    side = int(np.ceil(np.sqrt(N)))
+    grid = np.array([(a, c) for a in range(side) for c in range(side)][:N], dtype=float) * 4.0
+    centres = grid - grid.mean(axis=0) + rng.uniform(-0.5, 0.5, size=(N, 2))
+    x = np.repeat(centres[None], P, axis=0) + rng.normal(0, 0.3, size=(P, N, 2))
+    p = rng.normal(0, 0.5, size=(P, N, 2))
+    return kw, x, p
+
+
+def test_batch_exchange_device_workspace_2d_dipole():
+    """D=2 dipole replicas on two 512-thread workgroups per CU: N=64 (LDS)
+    beside N=700 (exchange workspace in device memory, about 11.6k doubles
+    against 10.2k), one with the normal-mode thermostat: forces against the
+    oracle per bead at 1e-10, then 10 device-noise steps against the
+    single-system context."""
+    cus = G.device_cus()
+    reps = [lattice_dipole_ring(64, 8, 2.0, 81, 0), lattice_dipole_ring(700, 4, 2.0, 82, 1)]
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    b.configure(512, 2 * cus)
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    errs = [per_bead_err(b.state(r)[2], O.update_forces(O.make_params(**kw), x)) for r, (kw, x, p) in enumerate(reps)]
+    print("2d dipole per-replica force errors", errs, flush=True)
+    assert max(errs) < 1e-10, errs
+    ctxs = []
+    for kw, x, p in reps:
+        c = G.Context(G.make_params(**kw))
+        c.set_state(x, p)
+        c.forces()
+        ctxs.append(c)
+    b.step(10)
+    for r, c in enumerate(ctxs):
+        c.step(10)
+        xb, pb, fb = b.state(r)
+        xc, pc, fc = c.state()
+        print("2d dipole", r, "x %.2e p %.2e f %.2e" % (rel_err(xb, xc), rel_err(pb, pc), per_bead_err(fb, fc)),
+              flush=True)
         assert rel_err(xb, xc) < 1e-10, r
         assert rel_err(pb, pc) < 1e-9, r
         assert per_bead_err(fb, fc) < 1e-9, r
