@@ -311,23 +311,35 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         const int q = lane / 16, c16 = lane % 16;
         const int half = (P + 1) / 2, width = G * 16, npair = (Pp / 2) * width;
         // c2 xi of the round starting at column block cb into buf (every thread its share)
+        // pair e = m width + gc of a round (modes 2m, 2m+1 of column gc)
+        auto pair_noise = [&](int m, int gc, int c0n, lds_double* buf) {
+            const int col = c0n + gc;
+            double g0 = 0.0, g1 = 0.0;
+            if (m < half && col < ncols) {
+                gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                if (pend) {   // c1 xi2 + xi1
+                    double h0, h1;
+                    gauss2((uint32_t)((size_t)m * ncols + col), step, 1u, r.rid, r.seed, h0, h1);
+                    g0 = fma(r.c1, h0, g0);
+                    g1 = fma(r.c1, h1, g1);
+                }
+            }
+            lds_double* out = buf + (size_t)(gc / 16) * Pp * 16 + gc % 16;
+            out[(2 * m) * 16] = 2 * m < P ? r.c2 * g0 : 0.0;
+            out[(2 * m + 1) * 16] = 2 * m + 1 < P ? r.c2 * g1 : 0.0;
+        };
+        // When the round's width divides the workgroup (16 G columns, G =
+        // waves / mode blocks: the usual case), a thread keeps one column and
+        // steps m by blockDim / width, the same pairs in the same order
+        // without a division per pair.
+        const bool colfixed = blockDim.x % width == 0;
+        const int gcf = threadIdx.x % width, m0f = threadIdx.x / width, mstep = blockDim.x / width;
         auto gen_noise = [&](int cb, lds_double* buf) {
             const int c0n = cb * 16;
-            for (int e = threadIdx.x; e < npair; e += blockDim.x) {
-                const int m = e / width, gc = e % width, col = c0n + gc;
-                double g0 = 0.0, g1 = 0.0;
-                if (m < half && col < ncols) {
-                    gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
-                    if (pend) {   // c1 xi2 + xi1
-                        double h0, h1;
-                        gauss2((uint32_t)((size_t)m * ncols + col), step, 1u, r.rid, r.seed, h0, h1);
-                        g0 = fma(r.c1, h0, g0);
-                        g1 = fma(r.c1, h1, g1);
-                    }
-                }
-                lds_double* out = buf + (size_t)(gc / 16) * Pp * 16 + gc % 16;
-                out[(2 * m) * 16] = 2 * m < P ? r.c2 * g0 : 0.0;
-                out[(2 * m + 1) * 16] = 2 * m + 1 < P ? r.c2 * g1 : 0.0;
+            if (colfixed) {
+                for (int m = m0f; m < Pp / 2; m += mstep) pair_noise(m, gcf, c0n, buf);
+            } else {
+                for (int e = threadIdx.x; e < npair; e += blockDim.x) pair_noise(e / width, e % width, c0n, buf);
             }
         };
         // two noise tiles: a round's products read one while the next round's
