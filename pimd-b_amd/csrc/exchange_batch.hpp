@@ -190,6 +190,16 @@ __device__ __forceinline__ void xb_prefix(const double* x1, const double* xP, in
 // barriers per block of 64 targets).  Writes U[1..N] and the records' U terms;
 // sets *status when a potential is not finite (the reference throws
 // overflow_error, quadratic_bosonic_exchange.cpp:92-97).
+//
+// A block's sources q < n0 are split in two.  Those of the blocks before the
+// previous one were taken early: while wave 0 ran the previous block's
+// min-plus pass and substitution, waves 1 .. nw-1 summed them for this
+// block's targets (half beside each pass), and each keeps its running
+// (m, s) in registers.  At the start of the block every wave takes its
+// share of the previous block's own targets, whose U was not known then,
+// continuing from those registers; the per-wave partials then enter the
+// min-plus shift and the sum as before.  The barriers per block stay four
+// and the layout is unchanged.
 template <int D, bool PBC, int UNROLL>
 __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay, double* lds, int* status) {
     constexpr int FR = xb_frec(D);
@@ -208,16 +218,80 @@ __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay
     double* psum = lds + lay.psum;
     double* a = lds + lay.tri;
     const int nblk = rec_blocks(N);
+    const bool early = nw > 1;     // waves besides the one running the passes
+    // The lane's target of the block starting at n0 (clamped past the block's
+    // end): xP[n-1], H_{n-1}, log2 n + c2 L_{n-1}.
+    struct Tgt {
+        double x[D], h, T;
+    };
+    auto target = [&](int n0, int nb) {
+        Tgt g;
+        const int tn = n0 + min(lane, nb - 1);
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) g.x[ax] = xP[ax * N + tn - 1];
+        g.h = Shi[tn - 1];
+        g.T = fma(c2, Slo[tn - 1], lg2[tn]);
+        return g;
+    };
+    // sources [qlo, qhi) into the lane's running minimum m of y and
+    // s = sum 2^(m - y), in groups of UNROLL (the last group padded with
+    // y = +inf, whose 2^(m - y) is taken as 2^-2000 = 0), rescaled when a
+    // group lowers m
+    auto accumulate = [&](const Tgt& g, int qlo, int qhi, double& m, double& s) {
+        auto ysrc = [&](int q) {
+            const double* fr = frec + q * FR;
+            double d[D];
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(g.x[ax] - fr[ax]);
+            return fma(c2, (g.h - fr[D]) + norm2_fma<D>(d), g.T + fr[D + 1]);
+        };
+        int q0 = qlo;
+        for (; q0 + UNROLL <= qhi; q0 += UNROLL) {
+            // the group's records first (all loads in flight together)
+            double rv[UNROLL][D + 2];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+                for (int f = 0; f < D + 2; ++f) rv[u][f] = frec[(q0 + u) * FR + f];
+            double y[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                double d[D];
+#pragma unroll
+                for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(g.x[ax] - rv[u][ax]);
+                y[u] = fma(c2, (g.h - rv[u][D]) + norm2_fma<D>(d), g.T + rv[u][D + 1]);
+            }
+            double cm = y[0];
+#pragma unroll
+            for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
+            const double mn = fmin(m, cm);
+            s *= exp2_pair(fmax(mn - m, -2000.0));   // m = +inf on the first group: s = 0 stays 0
+            m = mn;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) s += exp2_pair(mn - y[u]);
+        }
+        if (q0 < qhi) {   // the last, partial group
+            double y[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) y[u] = q0 + u < qhi ? ysrc(q0 + u) : INFINITY;
+            double cm = y[0];
+#pragma unroll
+            for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
+            const double mn = fmin(m, cm);
+            s *= exp2_pair(fmax(mn - m, -2000.0));
+            m = mn;
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) s += exp2_pair(fmax(mn - y[u], -2000.0));
+        }
+    };
+    double em = INFINITY, es = 0.0;   // waves >= 1: this wave's early partials for the next block's targets
     for (int blk = 0; blk < nblk; ++blk) {
         const int n0 = 1 + blk * REC_B;
         const int nb = min(REC_B, N - n0 + 1);
-        // the lane's target n (clamped past the block's end): xP[n-1], H_{n-1},
-        // log2 n + c2 L_{n-1}
-        const int tn = n0 + min(lane, nb - 1);
-        double xt[D];
-#pragma unroll
-        for (int ax = 0; ax < D; ++ax) xt[ax] = xP[ax * N + tn - 1];
-        const double th = Shi[tn - 1], tT = fma(c2, Slo[tn - 1], lg2[tn]);
+        const Tgt tg = target(n0, nb);
+        // sources of this block taken early (blocks before the previous one)
+        const bool had_early = early && blk >= 1;
+        const int lo1 = had_early ? n0 - REC_B : 0;
         // (0) in-block weights y(n0+t, n0+j) - U_{n0+j}, j < t < nb, at
         // a[rec_idx(j, t)]; +inf for targets past the block's end
         for (int vr = wave; vr < REC_B / 2; vr += nw) {
@@ -233,73 +307,46 @@ __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay
                 a[rec_idx(j, t)] = w;
             }
         }
-        // (1) this wave's share of the earlier sources q < n0 (a balanced
-        // contiguous range), in groups of UNROLL (the last group padded with
-        // y = +inf, whose 2^(m - y) is taken as 2^-2000 = 0): the lane's
-        // running minimum m of y and s = sum 2^(m - y), rescaled when a group
-        // lowers m
-        const int qlo = (int)(((long)n0 * wave) / nw), qhi = (int)(((long)n0 * (wave + 1)) / nw);
-        double m = INFINITY, s = 0.0;
-        // y of source q for this lane's target (q < qhi)
-        auto ysrc = [&](int q) {
-            const double* fr = frec + q * FR;
-            double d[D];
-#pragma unroll
-            for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(xt[ax] - fr[ax]);
-            return fma(c2, (th - fr[D]) + norm2_fma<D>(d), tT + fr[D + 1]);
-        };
-        if (lane < nb) {
-            int q0 = qlo;
-            for (; q0 + UNROLL <= qhi; q0 += UNROLL) {
-                // the group's records first (all loads in flight together)
-                double rv[UNROLL][D + 2];
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u)
-#pragma unroll
-                    for (int f = 0; f < D + 2; ++f) rv[u][f] = frec[(q0 + u) * FR + f];
-                double y[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    double d[D];
-#pragma unroll
-                    for (int ax = 0; ax < D; ++ax) d[ax] = xb_image<PBC>(xt[ax] - rv[u][ax]);
-                    y[u] = fma(c2, (th - rv[u][D]) + norm2_fma<D>(d), tT + rv[u][D + 1]);
-                }
-                double cm = y[0];
-#pragma unroll
-                for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
-                const double mn = fmin(m, cm);
-                s *= exp2_pair(fmax(mn - m, -2000.0));   // m = +inf on the first group: s = 0 stays 0
-                m = mn;
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) s += exp2_pair(mn - y[u]);
+        // (1) this wave's share of the sources [lo1, n0) (a balanced
+        // contiguous range)
+        {
+            const int span = n0 - lo1;
+            const int qlo = lo1 + (int)(((long)span * wave) / nw), qhi = lo1 + (int)(((long)span * (wave + 1)) / nw);
+            double m = INFINITY, s = 0.0;
+            if (had_early && wave >= 1) {   // continue from the early sources
+                m = em;
+                s = es;
             }
-            if (q0 < qhi) {   // the last, partial group
-                double y[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) y[u] = q0 + u < qhi ? ysrc(q0 + u) : INFINITY;
-                double cm = y[0];
-#pragma unroll
-                for (int u = 1; u < UNROLL; ++u) cm = fmin(cm, y[u]);
-                const double mn = fmin(m, cm);
-                s *= exp2_pair(fmax(mn - m, -2000.0));
-                m = mn;
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) s += exp2_pair(fmax(mn - y[u], -2000.0));
-            }
+            if (lane < nb) accumulate(tg, qlo, qhi, m, s);
+            part[wave * REC_B + lane] = m;
+            psum[wave * REC_B + lane] = s;
         }
-        part[wave * REC_B + lane] = m;
-        psum[wave * REC_B + lane] = s;
         __syncthreads();
+        // the next block's early sources [0, n0) on waves 1 .. nw-1, each
+        // wave's range in two halves (beside the min-plus pass, then beside
+        // the substitution)
+        const bool give = early && blk + 1 < nblk;
+        const int n1 = n0 + REC_B;
+        const int nb1 = give ? min(REC_B, N - n1 + 1) : 0;
+        em = INFINITY;
+        es = 0.0;
+        int elo = 0, emid = 0, ehi = 0;
+        if (give && wave >= 1) {
+            elo = (int)(((long)n0 * (wave - 1)) / (nw - 1));
+            ehi = (int)(((long)n0 * wave) / (nw - 1));
+            emid = elo + (ehi - elo) / 2;
+        }
         // (1b) in-block min-plus on wave 0: R_t = min(min_w m_w, min_j a[j][t] + R_j)
         if (wave == 0) {
             double rr = INFINITY;
             for (int w = 0; w < nw; ++w) rr = fmin(rr, part[w * REC_B + lane]);
             if (lane >= nb) rr = INFINITY;
             rec_inblock_seq<0>(a, lane, rr, R);
+        } else if (give && lane < nb1) {
+            accumulate(target(n1, nb1), elo, emid, em, es);
         }
         __syncthreads();
-        // (2a) each wave's earlier-source sum relative to R_t
+        // (2a) each wave's earlier-source sums relative to R_t
         {
             const int t = lane;
             const double sw = psum[wave * REC_B + t];
@@ -329,6 +376,8 @@ __device__ __forceinline__ void xb_forward(int N, double c2, const XbLayout& lay
                 if (n < N) frec[n * FR + D + 1] = fma(-c2, Slo[n], u);
                 if (!isfinite(u)) atomicOr(status, 1);
             }
+        } else if (give && lane < nb1) {
+            accumulate(target(n1, nb1), emid, ehi, em, es);
         }
         __syncthreads();
     }
