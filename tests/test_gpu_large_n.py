@@ -278,3 +278,31 @@ def test_batch_exchange_device_workspace_2d_dipole():
         assert per_bead_err(fb, fc) < 1e-9, r
         c.close()
     b.close()
+
+
+def test_batch_device_workspace_autotune_and_phase_timing():
+    """The autotuner over every launch shape that fits (each re-plans the LDS
+    and re-allocates the device-memory exchange workspaces) and the phase
+    timer (the exchange phase through the kernel that reaches them) on a
+    batch holding an N=600 replica beside an N=256 one: the state it
+    restores gives the oracle's forces per bead at 1e-10."""
+    reps = []
+    for r, N in enumerate([256, 600]):
+        kw, x = aziz_config(N, 4, 1.5, replicate=False, seed=900 + r)
+        kw["seed"] = 41 + r
+        p = np.random.default_rng(90 + r).normal(0, 0.3, size=x.shape)
+        reps.append((kw, x, p))
+    b = G.Batch([G.make_params(**kw) for kw, _, _ in reps])
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    threads, resident = b.autotune(trial_steps=2)
+    print("autotune picked", threads, resident, flush=True)
+    for phase in range(4):
+        assert b.time_phase(phase, 2) > 0.0
+    for r, (kw, x, p) in enumerate(reps):
+        b.set_state(r, x, p)
+    b.forces()
+    errs = [per_bead_err(b.state(r)[2], O.update_forces(O.make_params(**kw), x)) for r, (kw, x, p) in enumerate(reps)]
+    print("after autotune: per-replica force errors", errs, flush=True)
+    assert max(errs) < 1e-10, errs
+    b.close()
