@@ -236,6 +236,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
         const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
         const int wave = uni((int)(threadIdx.x / WAVE));
         const int Pp = nm_pad16(P), MB = Pp / 16, KK = Pp / 4;
+        const double dt_m = r.dt / r.mass;   // the drift x += (dt / m) p, one division per replica
         const int G = nm_mfma_groups(P, nw), ntask = MB * G;
         const int NCB = (ncols + 15) / 16;
         lds_double* CTl = (lds_double*)lds;
@@ -409,7 +410,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                                 double w = pnew - cmo;
                                 w = w + half_dt * fv[v];
                                 p[e] = w;
-                                x[e] = xv[v] + r.dt * w / r.mass;
+                                x[e] = fma(dt_m, w, xv[v]);
                             } else {
                                 const double pk = kick ? pv[v] + half_dt * fv[v] : pv[v];
                                 const double pnew = r.c1 * pk + acc[v];
@@ -753,6 +754,7 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
     double* p = uni(bp.p) + r.off_x;
     const double* f = uni(bp.f) + r.off_x;
     const double half_dt = 0.5 * r.dt;
+    const double dt_m = r.dt / r.mass;   // x += (dt / m) p: one division per replica
     const int n = P * D * N, nthr = blockDim.x;
     for (int base = threadIdx.x; base < n; base += STREAM_UNROLL * nthr) {
         double pv[STREAM_UNROLL], fv[STREAM_UNROLL], xv[STREAM_UNROLL];
@@ -773,7 +775,7 @@ __device__ __noinline__ void wg_kick_drift(const RepDesc& r_, const BatchPtrs& b
                 if (r.fixcom) v -= cm[elem_axis<D>(e, N)];
                 v = v + half_dt * fv[u];
                 p[e] = v;
-                x[e] = xv[u] + r.dt * v / r.mass;
+                x[e] = fma(dt_m, v, xv[u]);
             }
         }
     }
