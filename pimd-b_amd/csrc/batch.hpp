@@ -272,8 +272,8 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             }
             for (int col = threadIdx.x; col < ncols; col += blockDim.x) {
                 double g0, g1, h0 = 0.0, h1;
-                gauss2((uint32_t)col, step, ostep, r.rid, r.seed, g0, g1);   // mode 0 of column col
-                if (pend) gauss2((uint32_t)col, step, 1u, r.rid, r.seed, h0, h1);
+                gauss2<true>((uint32_t)col, step, ostep, r.rid, r.seed, g0, g1);   // mode 0 of column col
+                if (pend) gauss2<true>((uint32_t)col, step, 1u, r.rid, r.seed, h0, h1);
                 const int ax = col / N;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -317,10 +317,10 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
             const int col = c0n + gc;
             double g0 = 0.0, g1 = 0.0;
             if (m < half && col < ncols) {
-                gauss2((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                gauss2<true>((uint32_t)((size_t)m * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
                 if (pend) {   // c1 xi2 + xi1
                     double h0, h1;
-                    gauss2((uint32_t)((size_t)m * ncols + col), step, 1u, r.rid, r.seed, h0, h1);
+                    gauss2<true>((uint32_t)((size_t)m * ncols + col), step, 1u, r.rid, r.seed, h0, h1);
                     g0 = fma(r.c1, h0, g0);
                     g1 = fma(r.c1, h1, g1);
                 }
@@ -453,7 +453,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                 if (e2 >= n2) continue;
                 const int ax = col / N;
                 double g[2] = {0.0, 0.0};
-                if (r.thermostat == 1) gauss2((uint32_t)e2, step, ostep, r.rid, r.seed, g[0], g[1]);
+                if (r.thermostat == 1) gauss2<true>((uint32_t)e2, step, ostep, r.rid, r.seed, g[0], g[1]);
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int b = 2 * m + h;
@@ -515,7 +515,7 @@ __device__ __noinline__ void wg_momenta(const RepDesc& r_, const BatchPtrs& bp, 
                     const int k = kb + h;
                     if (k < P) {
                         double g0 = 0.0, g1 = 0.0;
-                        if (valid) gauss2((uint32_t)((size_t)(k >> 1) * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
+                        if (valid) gauss2<true>((uint32_t)((size_t)(k >> 1) * ncols + col), step, ostep, r.rid, r.seed, g0, g1);
                         pn[k * WAVE + lane] = r.c1 * sm[h] + r.c2 * g0;
                         if (k + 1 < P) pn[(k + 1) * WAVE + lane] = r.c1 * sm[h + 1] + r.c2 * g1;
                     }

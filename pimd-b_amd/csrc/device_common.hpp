@@ -399,6 +399,10 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 // [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716,
 // by its Taylor series sum_k s^(2k+1)/(2k+1) through s^21 (the rest is below
 // 1e-18 relative); e ln 2 with ln 2 in the two parts of exp_minimax.
+// FAST: s from the hardware reciprocal estimate of m + 1 (about 2^-23
+// relative), one Newton step and one correction of the quotient, within an
+// ulp or so of the correctly rounded division.
+template <bool FAST = false>
 __device__ __forceinline__ double log_unit(double u) {
     const double LN2_HI = 6.93147180369123816490e-01;
     const double LN2_LO = 1.90821492927058770002e-10;
@@ -409,7 +413,16 @@ __device__ __forceinline__ double log_unit(double u) {
         e -= 1;
     }
     const double f = m - 1.0;
-    const double s = f / (2.0 + f);
+    double s;
+    if constexpr (FAST) {
+        const double d = 2.0 + f;
+        double rc = __builtin_amdgcn_rcp(d);
+        rc = fma(rc, fma(-d, rc, 1.0), rc);
+        s = f * rc;
+        s = fma(fma(-d, s, f), rc, s);
+    } else {
+        s = f / (2.0 + f);
+    }
     const double z = s * s;
     double p = 1.0 / 21.0;
     p = fma_sconst(p, z, 1.0 / 19.0);
@@ -462,6 +475,12 @@ __device__ __forceinline__ void sincos_2pi(double u, double& sn, double& cs) {
 
 // Two standard normal deviates from one Philox block (Box-Muller on 53-bit
 // uniforms in (0,1]).  Counter = (element, step, ostep|stream, replica).
+// FAST (the batched form): the same draws with log_unit<true> and the square
+// root as l / sqrt(l) from the hardware estimate and one third-order
+// correction, each within an ulp or so of the correctly rounded operation,
+// so the deviates agree with the other form's to a few units in the last
+// place.
+template <bool FAST = false>
 __device__ __forceinline__ void gauss2(uint32_t elem, uint32_t step, uint32_t ostep,
                                        uint32_t replica, uint32_t seed, double& g0, double& g1) {
     const U4 r = philox4x32_10(U4{elem, step, ostep, replica}, seed, 0x5EEDB05Eu);
@@ -470,7 +489,16 @@ __device__ __forceinline__ void gauss2(uint32_t elem, uint32_t step, uint32_t os
     const uint64_t b = ((uint64_t)r.z << 21) ^ (uint64_t)(r.w >> 11);
     const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) / TWO53;   // (0, 1]
     const double u2 = (double)(b & ((1ull << 53) - 1)) / TWO53;           // [0, 1)
-    const double rad = sqrt(-2.0 * log_unit(u1));
+    double rad;
+    if constexpr (FAST) {
+        const double l = -2.0 * log_unit<true>(u1);   // >= 0; 0 at u1 = 1
+        const double y0 = __builtin_amdgcn_rsq(l);
+        const double e = fma(-l, y0 * y0, 1.0);
+        const double y = fma(y0 * e, fma(e, 0.375, 0.5), y0);
+        rad = l > 0.0 ? l * y : 0.0;
+    } else {
+        rad = sqrt(-2.0 * log_unit(u1));
+    }
     double s, c;
     sincos_2pi(u2, s, c);
     g0 = rad * c;
