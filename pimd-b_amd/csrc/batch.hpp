@@ -877,6 +877,7 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
     const double* x = uni(bp.x) + r.off_x;
     double* f = uni(bp.f) + r.off_x;
     const bool pbc = r.pbc != 0;
+    const double invL_s = pbc ? 1.0 / r.L : 0.0;   // the springs' min image
     const size_t bs = (size_t)D * N;
     const int lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
@@ -957,8 +958,10 @@ __device__ __noinline__ void wg_bead_forces(const RepDesc& r_, const BatchPtrs& 
                 const double phys = fe[a] + (in_lds ? Fw[ip + a] : fo[e]);
                 double spring = 0.0;
                 if (!exchange_bead) {
-                    const double dp = disp(xv[a], xprev[e], pbc, r.L);
-                    const double dn = disp(xv[a], xnext[e], pbc, r.L);
+                    // (the reference's image, min_image, without its division
+                    // except next to a tie)
+                    const double dp = disp_fast(xv[a], xprev[e], pbc, r.L, invL_s);
+                    const double dn = disp_fast(xv[a], xnext[e], pbc, r.L, invL_s);
                     spring = r.k * (dp + dn);
                 }
                 fo[e] = exchange_bead ? (after_exch ? fo[e] + phys : phys) : spring + phys;
