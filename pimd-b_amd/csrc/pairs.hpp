@@ -203,15 +203,18 @@ __device__ __forceinline__ bool pair_grad_frac(const double (&xi)[D], const doub
         const double q = y * y, q2 = q * q, q4 = q2 * q2;
         double att = q4 * fma(fma(fp.b2v, q, fp.b1v), q, fp.b0);
         if (rs < fp.dl) {
-            const double t = fma(fp.dl, y, -1.0);
-            const double e2 = (t * -1.4426950408889634) * t, k2 = rint(e2);
+            // The reference's hard-core range r / rm in (1e-7, 0.01), where it
+            // keeps the repulsion alone, needs no test of its own: there
+            // t = D rm / r - 1 > 123 and F = 0 exactly (F > 0 needs t < 27.4;
+            // the exponent takes t clamped to 38, e2 >= -2084, which keeps
+            // the conversion to int in range and changes no nonzero F),
+            // while the other factors stay finite (y^8 < 1.2e63), so the
+            // damped term is +-0 and g = y T1 to the bit, as with att = 0.
+            const double t = fma(fp.dl, y, -1.0), tc = fmin(t, 38.0);
+            const double e2 = (tc * -1.4426950408889634) * tc, k2 = rint(e2);
             const double F = ldexp(frac_exp2(e2 - k2), (int)k2) * (1.0 / AZ_T1C);
             const double pc = fma(fma(fp.b2 * 0.2, q, fp.b1 * 0.25), q, fp.b0 * (1.0 / 3.0)) * fp.dl;
-            const double damped = fma(-(t * y) * q4, pc, att) * F;
-            // r / rm = rs / a and a = dl / D: the reference's hard-core range
-            // (1e-7, 0.01) keeps the repulsion alone
-            constexpr double HLO = HARD_CORE_EPS / Aziz::D, HHI = 0.01 / Aziz::D;
-            att = (rs > HLO * fp.dl && rs < HHI * fp.dl) ? 0.0 : damped;
+            att = fma(-(t * y) * q4, pc, att) * F;
         }
         g = fma(y, T1, att);
     } else {

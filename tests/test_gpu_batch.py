@@ -158,6 +158,47 @@ def test_batch_forces_match_oracle(which):
     b.close()
 
 
+def test_batch_aziz_ranges_periodic():
+    """Aziz pairs in each range of the force under periodic boundaries with a
+    cutoff (the batched walk in box-fractional coordinates): the hard-core
+    range r/rm in (1e-7, 0.01) (repulsive term only, aziz.cpp:56-106), the
+    damped range on both sides of 0.01 and just below D, just above D and the
+    undamped tail (0.03 rm: the damping exponent below its clamp), each pair
+    60 bohr from the next; a distinguishable and a bosonic replica in one
+    batch, every pair's force against the oracle."""
+    rm = 5.60738
+    seps = np.array([0.005, 0.0099, 0.0101, 0.03, 0.3, 0.9, 1.2413, 1.2414, 1.6, 2.5]) * rm
+    n, P = len(seps), 3
+    rng = np.random.default_rng(23)
+    x = np.zeros((P, 2 * n, 3))
+    for k, s in enumerate(seps):
+        c = np.array([60.0 * k + 5.0, 7.0, 3.0])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for b in range(P):
+            x[b, 2 * k] = c
+            x[b, 2 * k + 1] = c + s * u
+    reps = []
+    for bosonic in (0, 1):
+        kw = dict(natoms=2 * n, nbeads=P, ndim=3, mass=4.0026 * AMU, temperature=2.0 * KELVIN, dt=20.0,
+                  gamma=1e-3, bosonic=bosonic, pbc=1, size=700.0, ext_kind="free", int_kind="aziz",
+                  cutoff=250.0, propagator="cartesian", thermostat="langevin", seed=3 + bosonic)
+        reps.append((kw, x))
+    b = G.Batch([G.make_params(**kw) for kw, _ in reps])
+    for r, (kw, xr) in enumerate(reps):
+        b.set_state(r, xr, np.zeros_like(xr))
+    b.forces()
+    for r, (kw, xr) in enumerate(reps):
+        _, _, f = b.state(r)
+        fo = O.update_forces(O.make_params(**kw), xr)
+        assert np.all(np.isfinite(f)), r
+        for k in range(n):
+            sl = slice(2 * k, 2 * k + 2)
+            err = np.max(np.abs(f[:, sl] - fo[:, sl])) / np.max(np.abs(fo[:, sl]))
+            assert err < 1e-10, (r, seps[k] / rm, err)
+    b.close()
+
+
 @pytest.mark.parametrize("which", ["dipole", "aziz", "nm", "nmprop", "nh", "nhnm"])
 def test_batch_follows_single_system(which):
     reps = {"dipole": lambda: dipole_replicas(6, seed=5), "aziz": lambda: aziz_replicas(4),
