@@ -356,7 +356,9 @@ struct SliceWalk {
     const lds_int* perm;
     int lane;
     double xj[D];   // the lane's column: positions,
-    unsigned jd;    // and its first element in F (kc S); N S or above: none
+    unsigned jd;    // and the LDS byte address of its first element in F
+                    // (F + kc S, so the updates need no address arithmetic);
+                    // at or past F + N S: none
 
     __device__ __forceinline__ bool load_row(int k, double (&xr)[D]) const {
         const bool ok = k < N;
@@ -373,13 +375,14 @@ struct SliceWalk {
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
+    __device__ __forceinline__ static unsigned lds_addr(const lds_double* q) { return (unsigned)(size_t)q; }
     __device__ __forceinline__ void load_col(int kc) {
         const int jc = kc < N ? perm[kc] : N;
-        jd = (unsigned)(kc * pair_fstride(D));
+        jd = lds_addr(F + kc * pair_fstride(D));
 #pragma unroll
         for (int a = 0; a < D; ++a) xj[a] = jc < N ? wg.load(xs, a * N + jc) : 0.0;
     }
-    __device__ __forceinline__ bool col_ok() const { return jd < (unsigned)(N * pair_fstride(D)); }
+    __device__ __forceinline__ bool col_ok() const { return jd < lds_addr(F + N * pair_fstride(D)); }
     __device__ __forceinline__ void rotate() {
 #pragma unroll
         for (int a = 0; a < D; ++a) xj[a] = lane_rotate(xj[a]);
@@ -390,7 +393,7 @@ struct SliceWalk {
         if (act) {
             double d[D], g;
             if (wg(xi, xj, d, g)) {
-                lds_double* Fj = F + jd;
+                lds_double* Fj = (lds_double*)(size_t)jd;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     ri[a] = fma(-g, d[a], ri[a]);
@@ -409,7 +412,7 @@ struct SliceWalk {
             if (ina || inb) {
                 if (!ina) ga = 0.0;
                 if (!inb) gb = 0.0;
-                lds_double* Fj = F + jd;
+                lds_double* Fj = (lds_double*)(size_t)jd;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     ra[a] = fma(-ga, da[a], ra[a]);
